@@ -1,0 +1,1218 @@
+// alpacc_hip.hip - MI355X (gfx950, CDNA4, wave64) lexer and LLP parser for alpacc grammars.
+//
+// Device path for the runtime of the reference (futhark/lexer.fut, futhark/parser.fut):
+//
+//   lexer : one persistent kernel. Every wavefront owns a 2 KiB tile (64 lanes x 32 bytes) and
+//           runs the whole pipeline for it: byte classes, a serial endomorphism walk per lane
+//           with the (endo x class) table in LDS, a wave64 Kogge-Stone scan of the lane
+//           endomorphisms with the (endo x endo) table, a decoupled look-back across tiles,
+//           then two walks of the plain lexer DFA from the now known lane start states: one to
+//           count token ends and token starts, one (after a second look-back over the counts)
+//           to write (kind, start, end) of every token. Input bytes are read from HBM once.
+//   parser: pair-table lookup per token window, segmented copy of bracket / production spans,
+//           bracket depth scan, bracket matching and parents through a 64-ary min-tree
+//           "previous smaller-or-equal" search, terminal-node substitution.
+//
+// All arithmetic is integer. Tables are produced by alpacc_amd/gen (the restated generator).
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/alpacc_hip.h"
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int LEX_IPT = 32;                      // bytes per lane
+constexpr int LEX_TILE = WAVE * LEX_IPT;         // bytes per wavefront tile
+constexpr int LEX_BLOCK = 1024;                  // 16 wavefronts share one LDS table copy
+constexpr uint32_t ST_AGG = 1u << 30;
+constexpr uint32_t ST_INC = 2u << 30;
+constexpr uint64_t CT_AGG = 1ull << 62;
+constexpr uint64_t CT_INC = 2ull << 62;
+constexpr uint32_t SPIN_LIMIT = 1u << 22;        // bounded waits: a stuck look-back ends the kernel
+
+// ----------------------------------------------------------------------------------------------
+// small device helpers
+
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_relaxed64(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed64(uint64_t *p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
+  return v;
+}
+__device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, WAVE));
+  return v;
+}
+
+struct LexArgs {
+  const uint8_t *in;
+  uint64_t n;
+  uint32_t *kinds;
+  uint32_t *starts;
+  uint32_t *ends;
+  const uint8_t *byte_class;
+  const uint16_t *endo_step;
+  const uint16_t *endo_comp;
+  const uint8_t *endo_apply;
+  const uint16_t *dfa_step;
+  const uint8_t *state_accept;
+  uint32_t e, C, m1, init;
+  int32_t ignore;
+  uint32_t num_tiles;
+  uint32_t aligned;           // input pointer is 16-byte aligned
+  uint32_t *tile_counter;     // [1]
+  uint32_t *st_state;         // [num_tiles]
+  uint64_t *st_count;         // [num_tiles]
+  uint32_t *result;           // [0] accept, [1] tokens, [2] timeout flag
+};
+
+__device__ __forceinline__ uint32_t comp_lookup(const uint16_t *tab, uint32_t e, uint32_t a,
+                                                uint32_t b) {
+  return tab[a * e + b];
+}
+
+// Exclusive prefix endomorphism of `tile` from the state look-back window (decoupled look-back,
+// combined strictly in tile order because composition is not commutative).
+__device__ uint32_t lookback_state(const LexArgs &p, const uint16_t *comp_tab, uint32_t tile,
+                                   int lane) {
+  uint32_t prefix = 0;  // identity
+  int64_t j = (int64_t)tile - 1;
+  uint32_t spins = 0;
+  while (true) {
+    const int64_t idx = j - lane;
+    const uint32_t w = idx >= 0 ? ld_relaxed(p.st_state + idx) : (ST_INC | 0u);
+    const uint32_t flag = w >> 30;
+    const uint64_t incm = __ballot(flag == 2);
+    const uint64_t zerom = __ballot(flag == 0);
+    const uint64_t upto = incm ? (((incm & (~incm + 1)) << 1) - 1) : ~0ull;
+    if (zerom & upto) {
+      if (++spins > SPIN_LIMIT || ld_relaxed(p.result + 2) != 0) {
+        if (lane == 0) st_relaxed(p.result + 2, 1u);
+        return 0;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    uint32_t v = ((upto >> lane) & 1ull) ? (w & 0xFFFFu) : 0u;
+    const int kmax = incm ? (__ffsll((long long)incm) - 1) : 63;
+    // order-preserving tree: lane l (later tile) absorbs lane l+d (earlier tiles) first
+    for (int d = 1; d <= kmax; d <<= 1) {
+      const uint32_t other = __shfl_down(v, d, WAVE);
+      if ((lane & (2 * d - 1)) == 0 && lane + d <= kmax) v = comp_lookup(comp_tab, p.e, other, v);
+    }
+    const uint32_t r = __shfl(v, 0, WAVE);
+    prefix = comp_lookup(comp_tab, p.e, r, prefix);
+    if (incm) return prefix;
+    j -= WAVE;
+  }
+}
+
+// Exclusive (token count, last token start) of `tile`; both combine commutatively.
+__device__ void lookback_count(const LexArgs &p, uint32_t tile, int lane, uint32_t &cnt,
+                               int32_t &last) {
+  uint32_t csum = 0;
+  int32_t lmax = -1;
+  int64_t j = (int64_t)tile - 1;
+  uint32_t spins = 0;
+  while (true) {
+    const int64_t idx = j - lane;
+    const uint64_t w = idx >= 0 ? ld_relaxed64(p.st_count + idx) : CT_INC;
+    const uint32_t flag = (uint32_t)(w >> 62);
+    const uint64_t incm = __ballot(flag == 2);
+    const uint64_t zerom = __ballot(flag == 0);
+    const uint64_t upto = incm ? (((incm & (~incm + 1)) << 1) - 1) : ~0ull;
+    if (zerom & upto) {
+      if (++spins > SPIN_LIMIT || ld_relaxed(p.result + 2) != 0) {
+        if (lane == 0) st_relaxed(p.result + 2, 1u);
+        cnt = 0;
+        last = -1;
+        return;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    const bool in = (upto >> lane) & 1ull;
+    uint32_t c = in ? (uint32_t)(w & 0x7FFFFFFFull) : 0u;
+    int32_t l = in ? (int32_t)((w >> 31) & 0x7FFFFFFFull) - 1 : -1;
+    csum += wave_sum_u32(c);
+    lmax = max(lmax, wave_max_i32(l));
+    if (incm) break;
+    j -= WAVE;
+  }
+  cnt = csum;
+  last = lmax;
+}
+
+__device__ __forceinline__ uint64_t pack_count(uint32_t cnt, int32_t last, uint64_t flag) {
+  return flag | ((uint64_t)(uint32_t)(last + 1) << 31) | (uint64_t)cnt;
+}
+
+#define LEX_CLS(j) ((cls[(j) >> 2] >> (8 * ((j)&3))) & 0xFFu)
+
+template <bool COMP_IN_LDS>
+__global__ void __launch_bounds__(LEX_BLOCK) lex_kernel(LexArgs p) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const uint32_t e = p.e, C = p.C, m1 = p.m1;
+  uint16_t *s_step = reinterpret_cast<uint16_t *>(smem);
+  uint16_t *s_dfa = s_step + e * C;
+  uint8_t *s_cls = reinterpret_cast<uint8_t *>(s_dfa + m1 * C);
+  uint16_t *s_comp = reinterpret_cast<uint16_t *>(s_cls + 256);
+  for (uint32_t i = threadIdx.x; i < e * C; i += blockDim.x) s_step[i] = p.endo_step[i];
+  for (uint32_t i = threadIdx.x; i < m1 * C; i += blockDim.x) s_dfa[i] = p.dfa_step[i];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_cls[i] = p.byte_class[i];
+  if (COMP_IN_LDS)
+    for (uint32_t i = threadIdx.x; i < e * e; i += blockDim.x) s_comp[i] = p.endo_comp[i];
+  __syncthreads();
+  const uint16_t *comp_tab = COMP_IN_LDS ? s_comp : p.endo_comp;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const uint64_t n = p.n;
+  const uint32_t ign = p.ignore < 0 ? 0xFFFFFFFFu : (uint32_t)p.ignore;
+
+  while (true) {
+    uint32_t tile = 0;
+    if (lane == 0) tile = atomicAdd(p.tile_counter, 1u);
+    tile = __shfl(tile, 0, WAVE);
+    if (tile >= p.num_tiles) break;
+    const uint64_t tbase = (uint64_t)tile * LEX_TILE;
+    const uint64_t base = tbase + (uint64_t)lane * LEX_IPT;
+    const int valid = base >= n ? 0 : (int)min((uint64_t)LEX_IPT, n - base);
+
+    // ---- bytes -> classes (packed 4 per register)
+    uint32_t cls[LEX_IPT / 4];
+    if (valid == LEX_IPT && p.aligned) {
+      const uint4 a = *reinterpret_cast<const uint4 *>(p.in + base);
+      const uint4 b = *reinterpret_cast<const uint4 *>(p.in + base + 16);
+      const uint32_t wds[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const uint32_t x = wds[q];
+        cls[q] = (uint32_t)s_cls[x & 0xFF] | ((uint32_t)s_cls[(x >> 8) & 0xFF] << 8) |
+                 ((uint32_t)s_cls[(x >> 16) & 0xFF] << 16) | ((uint32_t)s_cls[x >> 24] << 24);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int j = q * 4 + r;
+          if (j < valid) v |= (uint32_t)s_cls[p.in[base + j]] << (8 * r);
+        }
+        cls[q] = v;
+      }
+    }
+
+    // ---- pass 1: endomorphism of this lane's bytes
+    uint32_t a = 0;
+#pragma unroll
+    for (int j = 0; j < LEX_IPT; ++j)
+      if (j < valid) a = s_step[a * C + LEX_CLS(j)];
+
+    // ---- wave64 inclusive scan of endomorphisms (earlier lane first)
+    uint32_t incl = a;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+      const uint32_t other = __shfl_up(incl, d, WAVE);
+      if (lane >= d) incl = comp_lookup(comp_tab, e, other, incl);
+    }
+    const uint32_t agg = __shfl(incl, WAVE - 1, WAVE);
+    if (lane == 0) st_relaxed(p.st_state + tile, (tile == 0 ? ST_INC : ST_AGG) | agg);
+    uint32_t prefix = 0;
+    if (tile > 0) {
+      prefix = lookback_state(p, comp_tab, tile, lane);
+      if (lane == 0) st_relaxed(p.st_state + tile, ST_INC | comp_lookup(comp_tab, e, prefix, agg));
+    }
+
+    // ---- lane start states
+    uint32_t excl = __shfl_up(incl, 1, WAVE);
+    if (lane == 0) excl = 0;
+    const uint32_t s_tile = p.endo_apply[prefix * m1 + p.init];
+    const uint32_t s_lane = p.endo_apply[excl * m1 + s_tile];
+    uint32_t c_next = __shfl_down(LEX_CLS(0), 1, WAVE);
+    if (lane == WAVE - 1) {
+      const uint64_t nb = tbase + LEX_TILE;
+      c_next = nb < n ? s_cls[p.in[nb]] : 0u;
+    }
+
+    // ---- pass 2: count token ends and find the last token start
+    uint32_t cnt = 0;
+    int32_t last = -1;
+    uint32_t s_end = s_lane;
+    uint32_t lab_end = 0;
+    if (valid > 0) {
+      uint32_t ent = s_dfa[s_lane * C + LEX_CLS(0)];
+      uint32_t s = ent & 0xFFu;
+      uint32_t lab = ent >> 9;
+      if (((ent >> 8) & 1u) && lab != ign) last = (int32_t)base;
+#pragma unroll
+      for (int j = 0; j < LEX_IPT; ++j) {
+        if (j < valid) {
+          const bool inner = j + 1 < valid;
+          const bool has_next = inner || (base + j + 1 < n);
+          const uint32_t cn = inner ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
+          const uint32_t ent2 = has_next ? s_dfa[s * C + cn] : 0u;
+          const bool prod2 = (ent2 >> 8) & 1u;
+          const uint32_t lab2 = ent2 >> 9;
+          cnt += (has_next && prod2 && lab != ign) ? 1u : 0u;
+          if (inner) {
+            if (prod2 && lab2 != ign) last = (int32_t)(base + j + 1);
+            s = ent2 & 0xFFu;
+            lab = lab2;
+          }
+        }
+      }
+      s_end = s;
+      lab_end = lab;
+    }
+
+    // ---- wave scan of (count, last start)
+    uint32_t cincl = cnt;
+    int32_t lincl = last;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+      const uint32_t oc = __shfl_up(cincl, d, WAVE);
+      const int32_t ol = __shfl_up(lincl, d, WAVE);
+      if (lane >= d) {
+        cincl += oc;
+        lincl = max(lincl, ol);
+      }
+    }
+    const uint32_t cexcl = cincl - cnt;
+    int32_t lexcl = __shfl_up(lincl, 1, WAVE);
+    if (lane == 0) lexcl = -1;
+    const uint32_t ctot = __shfl(cincl, WAVE - 1, WAVE);
+    const int32_t ltot = __shfl(lincl, WAVE - 1, WAVE);
+    if (lane == 0) st_relaxed64(p.st_count + tile, pack_count(ctot, ltot, tile == 0 ? CT_INC : CT_AGG));
+    uint32_t cpre = 0;
+    int32_t lpre = -1;
+    if (tile > 0) {
+      lookback_count(p, tile, lane, cpre, lpre);
+      if (lane == 0) st_relaxed64(p.st_count + tile, pack_count(cpre + ctot, max(lpre, ltot), CT_INC));
+    }
+
+    // ---- pass 3: write tokens
+    uint32_t out = cpre + cexcl;
+    int32_t cur_start = max(max(lpre, lexcl), 0);
+    if (valid > 0) {
+      uint32_t ent = s_dfa[s_lane * C + LEX_CLS(0)];
+      uint32_t s = ent & 0xFFu;
+      uint32_t lab = ent >> 9;
+      if (((ent >> 8) & 1u) && lab != ign) cur_start = (int32_t)base;
+#pragma unroll
+      for (int j = 0; j < LEX_IPT; ++j) {
+        if (j < valid) {
+          const bool inner = j + 1 < valid;
+          const bool has_next = inner || (base + j + 1 < n);
+          const uint32_t cn = inner ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
+          const uint32_t ent2 = has_next ? s_dfa[s * C + cn] : 0u;
+          const bool prod2 = (ent2 >> 8) & 1u;
+          const uint32_t lab2 = ent2 >> 9;
+          if (has_next && prod2 && lab != ign) {
+            p.kinds[out] = lab;
+            p.starts[out] = (uint32_t)cur_start;
+            p.ends[out] = (uint32_t)(base + j + 1);
+            ++out;
+          }
+          if (inner) {
+            if (prod2 && lab2 != ign) cur_start = (int32_t)(base + j + 1);
+            s = ent2 & 0xFFu;
+            lab = lab2;
+          }
+        }
+      }
+      // ---- the lane holding the last byte appends the final token and decides acceptance
+      if (base + (uint64_t)valid == n) {
+        uint32_t total = out;
+        if (lab_end != ign) {
+          p.kinds[out] = lab_end;
+          p.starts[out] = (uint32_t)cur_start;
+          p.ends[out] = (uint32_t)n;
+          total = out + 1;
+        }
+        p.result[1] = total;
+        p.result[0] = p.state_accept[s_end] ? 1u : 0u;
+      }
+    }
+  }
+}
+
+__global__ void clear_u32(uint32_t *p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// ----------------------------------------------------------------------------------------------
+// device-wide scans of int2 (two independent int32 sums), reduce-then-scan
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+struct I2 {
+  int32_t x, y;
+};
+
+__device__ __forceinline__ I2 add2(I2 a, I2 b) { return I2{a.x + b.x, a.y + b.y}; }
+
+// block exclusive scan of one value per thread; returns the exclusive prefix, total in `total`
+__device__ I2 block_exclusive_scan(I2 v, I2 &total) {
+  __shared__ I2 wsum[SCAN_BLOCK / WAVE];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = threadIdx.x / WAVE;
+  I2 incl = v;
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    I2 o;
+    o.x = __shfl_up(incl.x, d, WAVE);
+    o.y = __shfl_up(incl.y, d, WAVE);
+    if (lane >= d) incl = add2(incl, o);
+  }
+  if (lane == WAVE - 1) wsum[w] = incl;
+  __syncthreads();
+  I2 woff{0, 0};
+  total = I2{0, 0};
+  for (int i = 0; i < SCAN_BLOCK / WAVE; ++i) {
+    if (i < w) woff = add2(woff, wsum[i]);
+    total = add2(total, wsum[i]);
+  }
+  __syncthreads();
+  I2 ex = add2(woff, incl);
+  ex.x -= v.x;
+  ex.y -= v.y;
+  return ex;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) scan_reduce(const I2 *in, uint64_t n, I2 *bsum) {
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  I2 s{0, 0};
+  for (int i = 0; i < SCAN_ITEMS; ++i)
+    if (base + i < n) s = add2(s, in[base + i]);
+  I2 total;
+  block_exclusive_scan(s, total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// one block: exclusive scan of the block sums in place; writes the grand total to bsum[nb]
+__global__ void __launch_bounds__(SCAN_BLOCK) scan_blocks(I2 *bsum, uint64_t nb) {
+  I2 carry{0, 0};
+  for (uint64_t off = 0; off < nb; off += SCAN_BLOCK) {
+    const uint64_t i = off + threadIdx.x;
+    const I2 v = i < nb ? bsum[i] : I2{0, 0};
+    I2 total;
+    const I2 ex = block_exclusive_scan(v, total);
+    if (i < nb) bsum[i] = add2(carry, ex);
+    carry = add2(carry, total);
+  }
+  if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) scan_down(const I2 *in, uint64_t n, const I2 *bsum,
+                                                        I2 *out, int inclusive) {
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  I2 vals[SCAN_ITEMS];
+  I2 s{0, 0};
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    vals[i] = base + i < n ? in[base + i] : I2{0, 0};
+    s = add2(s, vals[i]);
+  }
+  I2 total;
+  I2 run = add2(bsum[blockIdx.x], block_exclusive_scan(s, total));
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    if (base + i < n) {
+      if (inclusive) {
+        run = add2(run, vals[i]);
+        out[base + i] = run;
+      } else {
+        out[base + i] = run;
+        run = add2(run, vals[i]);
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------------------------
+// 64-ary min levels and the "previous smaller-or-equal" search (find_previous (<=),
+// futhark/parser.fut:92-108, over a 64-ary tree instead of a binary one)
+
+constexpr int MAX_LEVELS = 8;
+
+struct Levels {
+  const int32_t *L[MAX_LEVELS];
+  uint64_t len[MAX_LEVELS];
+  int nlev;
+};
+
+__global__ void min_reduce64(const int32_t *in, uint64_t n, int32_t *out, uint64_t nout) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nout;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    int32_t m = INT32_MAX;
+    const uint64_t lo = i * 64, hi = min(lo + 64, n);
+    for (uint64_t j = lo; j < hi; ++j) m = min(m, in[j]);
+    out[i] = m;
+  }
+}
+
+__device__ int64_t prev_le(const Levels &lv, uint64_t i, int32_t v) {
+  int lev = 0;
+  uint64_t pos = i;
+  int64_t found = -1;
+  while (true) {
+    const int32_t *A = lv.L[lev];
+    const uint64_t lo = (lev == lv.nlev - 1) ? 0 : ((pos >> 6) << 6);
+    for (uint64_t j = pos; j > lo; --j) {
+      if (A[j - 1] <= v) {
+        found = (int64_t)(j - 1);
+        break;
+      }
+    }
+    if (found >= 0) break;
+    if (lev == lv.nlev - 1) return -1;
+    pos >>= 6;
+    ++lev;
+  }
+  while (lev > 0) {
+    const int32_t *A = lv.L[lev - 1];
+    const uint64_t lo = (uint64_t)found * 64;
+    const uint64_t hi = min(lo + 64, lv.len[lev - 1]);
+    int64_t f = -1;
+    for (uint64_t j = hi; j > lo; --j) {
+      if (A[j - 1] <= v) {
+        f = (int64_t)(j - 1);
+        break;
+      }
+    }
+    found = f;
+    --lev;
+    if (found < 0) return -1;
+  }
+  return found;
+}
+
+// ----------------------------------------------------------------------------------------------
+// parser kernels
+
+struct ParseArgs {
+  const uint32_t *kinds;
+  uint64_t T;  // tokens
+  uint32_t q, k, nT, start_t, end_t, empty_t;
+  const uint32_t *pair_index;
+  const uint32_t *pair_entry;
+  const uint16_t *stacks;
+  const uint16_t *prods;
+  const int32_t *arity;
+  const int32_t *p2t;
+  uint32_t *flags;  // [0] reject
+};
+
+// keys over [start] ++ kinds ++ [end] (get_key / lookup, parser.fut:117-151)
+__global__ void parse_keys(ParseArgs a, uint32_t *ent, I2 *lens) {
+  const uint64_t P = a.T + 2;
+  for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < P;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t di = 0;
+    for (uint32_t j = 0; j < a.q + a.k; ++j) {
+      const int64_t pos = (int64_t)p + (int64_t)j - (int64_t)a.q;
+      uint32_t t;
+      if (pos < 0 || pos >= (int64_t)P) t = a.empty_t;
+      else if (pos == 0) t = a.start_t;
+      else if (pos == (int64_t)P - 1) t = a.end_t;
+      else t = a.kinds[pos - 1];
+      di = di * a.nT + (t < a.nT ? t : a.empty_t);
+    }
+    const uint32_t id = a.pair_index[di];
+    if (id == 0xFFFFFFFFu) {
+      atomicOr(a.flags, 1u);
+      ent[p] = 0;
+      lens[p] = I2{0, 0};
+    } else {
+      ent[p] = id;
+      const uint32_t *e4 = a.pair_entry + 4ull * id;
+      lens[p] = I2{(int32_t)(e4[1] - e4[0]), (int32_t)(e4[3] - e4[2])};
+    }
+  }
+}
+
+// segmented copy of stack and production spans (segmented_copy, parser.fut:216-233)
+__global__ void parse_expand(ParseArgs a, const uint32_t *ent, const I2 *offs, uint16_t *br,
+                             uint16_t *prod) {
+  const uint64_t P = a.T + 2;
+  for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < P;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t *e4 = a.pair_entry + 4ull * ent[p];
+    const I2 o = offs[p];
+    for (uint32_t s = e4[0]; s < e4[1]; ++s) br[(uint64_t)o.x + (s - e4[0])] = a.stacks[s];
+    for (uint32_t s = e4[2]; s < e4[3]; ++s) prod[(uint64_t)o.y + (s - e4[2])] = a.prods[s];
+  }
+}
+
+__global__ void bracket_vals(const uint16_t *br, uint64_t B, I2 *v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < B;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    v[i] = I2{(br[i] & 0x8000u) ? 1 : -1, 0};
+}
+
+// depths (parser.fut:153-166): reject on a negative prefix or a non-zero total
+__global__ void bracket_depth(const uint16_t *br, const I2 *incl, uint64_t B, int32_t *depth,
+                              uint32_t *flags) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < B;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const int32_t s = incl[i].x;
+    if (s < 0 || (i == B - 1 && s != 0)) atomicOr(flags, 1u);
+    depth[i] = s - ((br[i] & 0x8000u) ? 1 : 0);
+  }
+}
+
+// brackets_matches (parser.fut:174-184)
+__global__ void bracket_match(const uint16_t *br, uint64_t B, Levels lv, uint32_t *flags) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < B;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint16_t b = br[i];
+    if (b & 0x8000u) continue;
+    const int64_t j = prev_le(lv, i, lv.L[0][i]);
+    if (j < 0 || (br[j] & 0x7FFFu) != (b & 0x7FFFu)) atomicOr(flags, 1u);
+  }
+}
+
+__global__ void prod_vals(ParseArgs a, const uint16_t *prod, uint64_t N, I2 *v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint16_t pr = prod[i];
+    v[i] = I2{a.arity[pr] - 1, a.p2t[pr] >= 0 ? 1 : 0};
+  }
+}
+
+__global__ void split_x(const I2 *v, uint64_t N, int32_t *x) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    x[i] = v[i].x;
+}
+
+// parents (parser.fut:286-296) and terminal nodes (parser.fut:305-332)
+__global__ void parse_nodes(ParseArgs a, const uint16_t *prod, const I2 *ex, uint64_t N, Levels lv,
+                            const uint32_t *starts, const uint32_t *ends, uint32_t *parent,
+                            uint32_t *node, uint32_t *span) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint16_t pr = prod[i];
+    const int64_t par = i == 0 ? 0 : prev_le(lv, i, ex[i].x);
+    parent[i] = (uint32_t)par;
+    const int32_t t = a.p2t[pr];
+    if (t >= 0) {
+      const uint32_t r = (uint32_t)ex[i].y;
+      node[i] = ALPACC_NODE_TERMINAL | (uint32_t)t;
+      span[2 * i] = starts[r];
+      span[2 * i + 1] = ends[r];
+    } else {
+      node[i] = pr;
+      span[2 * i] = 0;
+      span[2 * i + 1] = 0;
+    }
+  }
+}
+
+inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 16) {
+  uint64_t g = (n + block - 1) / block;
+  if (g == 0) g = 1;
+  return (unsigned)std::min<uint64_t>(g, cap);
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------------------------
+// host side
+
+static std::string g_create_error;
+
+struct alpacc_ctx {
+  int device = 0;
+  int cus = 0;
+  std::string err;
+  int launches = 0;
+  // lexer tables
+  bool has_lexer = false;
+  uint32_t e = 0, C = 0, m1 = 0, init = 0, dead = 0;
+  int32_t ignore = -1;
+  uint8_t *d_cls = nullptr;
+  uint16_t *d_step = nullptr, *d_comp = nullptr, *d_dfa = nullptr;
+  uint8_t *d_apply = nullptr, *d_accept = nullptr;
+  bool comp_in_lds = false;
+  size_t lex_lds = 0;
+  // parser tables
+  bool has_parser = false;
+  uint32_t q = 0, k = 0, nT = 0, start_t = 0, end_t = 0, empty_t = 0, P = 0;
+  uint32_t *d_pair_index = nullptr, *d_pair_entry = nullptr;
+  uint16_t *d_stacks = nullptr, *d_prods = nullptr;
+  int32_t *d_arity = nullptr, *d_p2t = nullptr;
+  // scratch (grow-only)
+  std::vector<std::pair<void *, size_t>> scratch;
+  uint32_t *h_small = nullptr;  // pinned host words for results
+
+  void *buf(int id, size_t bytes) {
+    if ((int)scratch.size() <= id) scratch.resize(id + 1, {nullptr, 0});
+    auto &s = scratch[id];
+    if (s.second < bytes) {
+      if (s.first) (void)hipFree(s.first);
+      s.first = nullptr;
+      s.second = 0;
+      size_t want = std::max<size_t>(bytes, 256);
+      if (hipMalloc(&s.first, want) != hipSuccess) return nullptr;
+      s.second = want;
+    }
+    return s.first;
+  }
+};
+
+namespace {
+
+enum BufId {
+  B_SMALL = 0,
+  B_ST_STATE,
+  B_ST_COUNT,
+  B_TOK_KIND,
+  B_TOK_START,
+  B_TOK_END,
+  B_ENT,
+  B_LENS,
+  B_OFFS,
+  B_BSUM,
+  B_BR,
+  B_PROD,
+  B_BVAL,
+  B_BINCL,
+  B_DEPTH,
+  B_LEVELS,
+  B_PVAL,
+  B_PEX,
+  B_SX,
+};
+
+template <typename T>
+T *upload(const T *h, size_t count, std::string &err) {
+  if (count == 0) count = 1;
+  T *d = nullptr;
+  if (hipMalloc(&d, count * sizeof(T)) != hipSuccess) {
+    err = "hipMalloc failed for a table";
+    return nullptr;
+  }
+  if (h && hipMemcpy(d, h, count * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+    err = "hipMemcpy failed for a table";
+    return nullptr;
+  }
+  return d;
+}
+
+#define CK(expr)                                                        \
+  do {                                                                  \
+    hipError_t e_ = (expr);                                             \
+    if (e_ != hipSuccess) {                                             \
+      ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);      \
+      return ALPACC_ERR_HIP;                                            \
+    }                                                                   \
+  } while (0)
+
+int scan_i2(alpacc_ctx *ctx, const I2 *in, I2 *out, uint64_t n, bool inclusive, I2 *total,
+            hipStream_t st) {
+  const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  I2 *bsum = (I2 *)ctx->buf(B_BSUM, (nb + 1) * sizeof(I2));
+  if (!bsum) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  if (n > 0) {
+    scan_reduce<<<(unsigned)nb, SCAN_BLOCK, 0, st>>>(in, n, bsum);
+    scan_blocks<<<1, SCAN_BLOCK, 0, st>>>(bsum, nb);
+    scan_down<<<(unsigned)nb, SCAN_BLOCK, 0, st>>>(in, n, bsum, out, inclusive ? 1 : 0);
+    ctx->launches += 3;
+    CK(hipGetLastError());
+  }
+  if (total) {
+    if (n == 0) {
+      *total = I2{0, 0};
+    } else {
+      CK(hipMemcpyAsync(total, bsum + nb, sizeof(I2), hipMemcpyDeviceToHost, st));
+      CK(hipStreamSynchronize(st));
+    }
+  }
+  return 0;
+}
+
+int build_levels(alpacc_ctx *ctx, int32_t *base, uint64_t n, Levels &lv, hipStream_t st) {
+  uint64_t total = 0, len = n;
+  while (len > 64) {
+    len = (len + 63) / 64;
+    total += len;
+  }
+  int32_t *area = (int32_t *)ctx->buf(B_LEVELS, (total + 1) * sizeof(int32_t));
+  if (!area) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  lv.nlev = 1;
+  lv.L[0] = base;
+  lv.len[0] = n;
+  int32_t *cur = base;
+  len = n;
+  uint64_t off = 0;
+  while (len > 64) {
+    if (lv.nlev >= MAX_LEVELS) {
+      ctx->err = "input too large for the min levels";
+      return ALPACC_ERR_ARG;
+    }
+    const uint64_t nl = (len + 63) / 64;
+    int32_t *next = area + off;
+    min_reduce64<<<grid_for(nl, 256), 256, 0, st>>>(cur, len, next, nl);
+    ctx->launches++;
+    lv.L[lv.nlev] = next;
+    lv.len[lv.nlev] = nl;
+    lv.nlev++;
+    off += nl;
+    cur = next;
+    len = nl;
+  }
+  CK(hipGetLastError());
+  return 0;
+}
+
+int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, uint32_t *starts,
+            uint32_t *ends, uint64_t *num_tokens, hipStream_t st) {
+  if (!ctx->has_lexer) {
+    ctx->err = "bundle has no lexer";
+    return ALPACC_ERR_ARG;
+  }
+  if (n == 0) {  // the identity state never accepts (lexer.fut:170-172)
+    *num_tokens = 0;
+    return ALPACC_REJECT;
+  }
+  if (n >= (1ull << 31) - 2) {
+    ctx->err = "documents must be shorter than 2^31 - 2 bytes";
+    return ALPACC_ERR_ARG;
+  }
+  const uint64_t tiles = (n + LEX_TILE - 1) / LEX_TILE;
+  uint32_t *small = (uint32_t *)ctx->buf(B_SMALL, 64);
+  uint32_t *st_state = (uint32_t *)ctx->buf(B_ST_STATE, tiles * sizeof(uint32_t));
+  uint64_t *st_count = (uint64_t *)ctx->buf(B_ST_COUNT, tiles * sizeof(uint64_t));
+  if (!small || !st_state || !st_count) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  clear_u32<<<1, 64, 0, st>>>(small, 16);
+  clear_u32<<<grid_for(tiles, 256), 256, 0, st>>>(st_state, tiles);
+  clear_u32<<<grid_for(2 * tiles, 256), 256, 0, st>>>((uint32_t *)st_count, 2 * tiles);
+  LexArgs a;
+  a.in = d_in;
+  a.n = n;
+  a.kinds = kinds;
+  a.starts = starts;
+  a.ends = ends;
+  a.byte_class = ctx->d_cls;
+  a.endo_step = ctx->d_step;
+  a.endo_comp = ctx->d_comp;
+  a.endo_apply = ctx->d_apply;
+  a.dfa_step = ctx->d_dfa;
+  a.state_accept = ctx->d_accept;
+  a.e = ctx->e;
+  a.C = ctx->C;
+  a.m1 = ctx->m1;
+  a.init = ctx->init;
+  a.ignore = ctx->ignore;
+  a.num_tiles = (uint32_t)tiles;
+  a.aligned = ((uintptr_t)d_in & 15) == 0 ? 1u : 0u;
+  a.tile_counter = small;
+  a.st_state = st_state;
+  a.st_count = st_count;
+  a.result = small + 4;
+  const uint64_t waves_per_block = LEX_BLOCK / WAVE;
+  uint64_t blocks = (tiles + waves_per_block - 1) / waves_per_block;
+  blocks = std::min<uint64_t>(blocks, (uint64_t)std::max(1, ctx->cus) * 2);
+  if (ctx->comp_in_lds)
+    lex_kernel<true><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
+  else
+    lex_kernel<false><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
+  ctx->launches += 4;
+  CK(hipGetLastError());
+  CK(hipMemcpyAsync(ctx->h_small, small + 4, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+  if (ctx->h_small[2]) {
+    ctx->err = "lexer look-back did not complete";
+    return ALPACC_ERR_TIMEOUT;
+  }
+  *num_tokens = ctx->h_small[1];
+  return ctx->h_small[0] ? ALPACC_ACCEPT : ALPACC_REJECT;
+}
+
+int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *starts,
+                     const uint32_t *ends, uint64_t T, uint32_t *parent, uint32_t *node,
+                     uint32_t *span, uint64_t cap, uint64_t *num_nodes, hipStream_t st) {
+  if (!ctx->has_parser) {
+    ctx->err = "bundle has no parser";
+    return ALPACC_ERR_NO_PARSER;
+  }
+  *num_nodes = 0;
+  const uint64_t P = T + 2;
+  uint32_t *flags = (uint32_t *)ctx->buf(B_SMALL, 64) + 8;
+  uint32_t *ent = (uint32_t *)ctx->buf(B_ENT, P * sizeof(uint32_t));
+  I2 *lens = (I2 *)ctx->buf(B_LENS, P * sizeof(I2));
+  I2 *offs = (I2 *)ctx->buf(B_OFFS, P * sizeof(I2));
+  if (!flags || !ent || !lens || !offs) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  ParseArgs a;
+  a.kinds = kinds;
+  a.T = T;
+  a.q = ctx->q;
+  a.k = ctx->k;
+  a.nT = ctx->nT;
+  a.start_t = ctx->start_t;
+  a.end_t = ctx->end_t;
+  a.empty_t = ctx->empty_t;
+  a.pair_index = ctx->d_pair_index;
+  a.pair_entry = ctx->d_pair_entry;
+  a.stacks = ctx->d_stacks;
+  a.prods = ctx->d_prods;
+  a.arity = ctx->d_arity;
+  a.p2t = ctx->d_p2t;
+  a.flags = flags;
+  clear_u32<<<1, 64, 0, st>>>(flags, 4);
+  parse_keys<<<grid_for(P, 256), 256, 0, st>>>(a, ent, lens);
+  ctx->launches += 2;
+  CK(hipGetLastError());
+  I2 tot;
+  int rc = scan_i2(ctx, lens, offs, P, false, &tot, st);
+  if (rc) return rc;
+  CK(hipMemcpyAsync(ctx->h_small, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+  if (ctx->h_small[0]) return ALPACC_REJECT;  // a missing pair (to_keys, parser.fut:235-240)
+  const uint64_t B = (uint32_t)tot.x, N = (uint32_t)tot.y;
+  uint16_t *br = (uint16_t *)ctx->buf(B_BR, B * sizeof(uint16_t) + 2);
+  uint16_t *prod = (uint16_t *)ctx->buf(B_PROD, N * sizeof(uint16_t) + 2);
+  if (!br || !prod) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  parse_expand<<<grid_for(P, 256), 256, 0, st>>>(a, ent, offs, br, prod);
+  ctx->launches++;
+  // bracket depths and matching
+  if (B > 0) {
+    I2 *bval = (I2 *)ctx->buf(B_BVAL, B * sizeof(I2));
+    I2 *bincl = (I2 *)ctx->buf(B_BINCL, B * sizeof(I2));
+    int32_t *depth = (int32_t *)ctx->buf(B_DEPTH, B * sizeof(int32_t));
+    if (!bval || !bincl || !depth) {
+      ctx->err = "scratch allocation failed";
+      return ALPACC_ERR_HIP;
+    }
+    bracket_vals<<<grid_for(B, 256), 256, 0, st>>>(br, B, bval);
+    ctx->launches++;
+    rc = scan_i2(ctx, bval, bincl, B, true, nullptr, st);
+    if (rc) return rc;
+    bracket_depth<<<grid_for(B, 256), 256, 0, st>>>(br, bincl, B, depth, flags);
+    ctx->launches++;
+    Levels lv;
+    rc = build_levels(ctx, depth, B, lv, st);
+    if (rc) return rc;
+    bracket_match<<<grid_for(B, 256), 256, 0, st>>>(br, B, lv, flags);
+    ctx->launches++;
+    CK(hipGetLastError());
+  }
+  CK(hipMemcpyAsync(ctx->h_small, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+  if (ctx->h_small[0]) return ALPACC_REJECT;
+  *num_nodes = N;
+  if (N > cap) {
+    ctx->err = "output capacity too small";
+    return ALPACC_ERR_CAPACITY;
+  }
+  if (N == 0) return ALPACC_ACCEPT;
+  I2 *pval = (I2 *)ctx->buf(B_PVAL, N * sizeof(I2));
+  I2 *pex = (I2 *)ctx->buf(B_PEX, N * sizeof(I2));
+  int32_t *sx = (int32_t *)ctx->buf(B_SX, N * sizeof(int32_t));
+  if (!pval || !pex || !sx) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  prod_vals<<<grid_for(N, 256), 256, 0, st>>>(a, prod, N, pval);
+  ctx->launches++;
+  I2 ptot;
+  rc = scan_i2(ctx, pval, pex, N, false, &ptot, st);
+  if (rc) return rc;
+  if ((uint64_t)(uint32_t)ptot.y != T) {
+    ctx->err = "terminal productions do not match the token count";
+    return ALPACC_ERR_ARG;
+  }
+  split_x<<<grid_for(N, 256), 256, 0, st>>>(pex, N, sx);
+  ctx->launches++;
+  Levels lv;
+  rc = build_levels(ctx, sx, N, lv, st);
+  if (rc) return rc;
+  parse_nodes<<<grid_for(N, 256), 256, 0, st>>>(a, prod, pex, N, lv, starts, ends, parent, node,
+                                                span);
+  ctx->launches++;
+  CK(hipGetLastError());
+  CK(hipStreamSynchronize(st));
+  return ALPACC_ACCEPT;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *alpacc_create_error(void) { return g_create_error.c_str(); }
+
+alpacc_ctx *alpacc_create(const alpacc_tables *t, int device) {
+  g_create_error.clear();
+  if (!t) {
+    g_create_error = "null tables";
+    return nullptr;
+  }
+  if (hipSetDevice(device) != hipSuccess) {
+    g_create_error = "hipSetDevice failed";
+    return nullptr;
+  }
+  alpacc_ctx *ctx = new alpacc_ctx();
+  ctx->device = device;
+  hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device);
+  std::string &err = g_create_error;
+  if (hipHostMalloc((void **)&ctx->h_small, 64, hipHostMallocDefault) != hipSuccess) {
+    err = "hipHostMalloc failed";
+    delete ctx;
+    return nullptr;
+  }
+  if (t->num_endos > 0) {
+    if (t->num_endos > 65535 || t->num_states > 256 || t->num_classes > 256 ||
+        t->init_state >= t->num_states) {
+      err = "lexer tables out of range";
+      delete ctx;
+      return nullptr;
+    }
+    ctx->has_lexer = true;
+    ctx->e = t->num_endos;
+    ctx->C = t->num_classes;
+    ctx->m1 = t->num_states;
+    ctx->init = t->init_state;
+    ctx->ignore = t->ignore_terminal;
+    ctx->dead = t->dead_terminal;
+    ctx->d_cls = upload(t->byte_class, 256, err);
+    ctx->d_step = upload(t->endo_step, (size_t)ctx->e * ctx->C, err);
+    ctx->d_comp = upload(t->endo_comp, (size_t)ctx->e * ctx->e, err);
+    ctx->d_apply = upload(t->endo_apply, (size_t)ctx->e * ctx->m1, err);
+    ctx->d_dfa = upload(t->dfa_step, (size_t)ctx->m1 * ctx->C, err);
+    ctx->d_accept = upload(t->state_accept, ctx->m1, err);
+    if (!err.empty()) {
+      delete ctx;
+      return nullptr;
+    }
+    size_t lds = ((size_t)ctx->e * ctx->C + (size_t)ctx->m1 * ctx->C) * 2 + 256;
+    lds = (lds + 15) & ~(size_t)15;
+    const size_t comp_bytes = (size_t)ctx->e * ctx->e * 2;
+    ctx->comp_in_lds = lds + comp_bytes <= 48 * 1024;
+    if (ctx->comp_in_lds) lds += comp_bytes;
+    if (lds > 64 * 1024) {
+      const int want = (int)lds;
+      if (hipFuncSetAttribute((const void *)lex_kernel<false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess) {
+        err = "lexer tables do not fit in LDS";
+        delete ctx;
+        return nullptr;
+      }
+    }
+    if (lds > 160 * 1024) {
+      err = "lexer tables do not fit in LDS";
+      delete ctx;
+      return nullptr;
+    }
+    ctx->lex_lds = lds;
+  }
+  if (t->num_productions > 0 && t->pair_index) {
+    ctx->has_parser = true;
+    ctx->q = t->q;
+    ctx->k = t->k;
+    ctx->nT = t->num_terminals;
+    ctx->start_t = t->start_terminal;
+    ctx->end_t = t->end_terminal;
+    ctx->empty_t = t->empty_terminal;
+    ctx->P = t->num_productions;
+    ctx->d_pair_index = upload(t->pair_index, t->pair_index_len, err);
+    ctx->d_pair_entry = upload(t->pair_entry, (size_t)t->num_pair_entries * 4, err);
+    ctx->d_stacks = upload(t->stacks, t->stacks_len, err);
+    ctx->d_prods = upload(t->productions, t->productions_len, err);
+    ctx->d_arity = upload(t->arities, t->num_productions, err);
+    ctx->d_p2t = upload(t->prod_to_terminal, t->num_productions, err);
+    if (!err.empty()) {
+      delete ctx;
+      return nullptr;
+    }
+  }
+  return ctx;
+}
+
+void alpacc_destroy(alpacc_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  for (auto &s : ctx->scratch)
+    if (s.first) (void)hipFree(s.first);
+  void *tabs[] = {ctx->d_cls,          ctx->d_step,         ctx->d_comp,   ctx->d_apply,
+                  ctx->d_dfa,          ctx->d_accept,       ctx->d_pair_index, ctx->d_pair_entry,
+                  ctx->d_stacks,       ctx->d_prods,        ctx->d_arity,  ctx->d_p2t};
+  for (void *p : tabs)
+    if (p) (void)hipFree(p);
+  if (ctx->h_small) (void)hipHostFree(ctx->h_small);
+  delete ctx;
+}
+
+const char *alpacc_last_error(alpacc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int alpacc_last_launch_count(alpacc_ctx *ctx) { return ctx ? ctx->launches : 0; }
+
+int alpacc_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_kinds,
+               uint32_t *d_starts, uint32_t *d_ends, uint64_t cap, uint64_t *num_tokens,
+               void *stream) {
+  if (!ctx || !num_tokens || (n > 0 && (!d_in || !d_kinds || !d_starts || !d_ends))) return ALPACC_ERR_ARG;
+  if (cap < n) {
+    ctx->err = "token capacity must be at least the input length";
+    return ALPACC_ERR_CAPACITY;
+  }
+  ctx->launches = 0;
+  ctx->err.clear();
+  (void)hipSetDevice(ctx->device);
+  return run_lex(ctx, d_in, n, d_kinds, d_starts, d_ends, num_tokens, (hipStream_t)stream);
+}
+
+int alpacc_parse_tokens(alpacc_ctx *ctx, const uint32_t *d_kinds, const uint32_t *d_starts,
+                        const uint32_t *d_ends, uint64_t num_tokens, uint32_t *d_parent,
+                        uint32_t *d_node, uint32_t *d_span, uint64_t cap, uint64_t *num_nodes,
+                        void *stream) {
+  if (!ctx || !num_nodes) return ALPACC_ERR_ARG;
+  ctx->launches = 0;
+  ctx->err.clear();
+  (void)hipSetDevice(ctx->device);
+  return run_parse_tokens(ctx, d_kinds, d_starts, d_ends, num_tokens, d_parent, d_node, d_span,
+                          cap, num_nodes, (hipStream_t)stream);
+}
+
+int alpacc_parse(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_parent,
+                 uint32_t *d_node, uint32_t *d_span, uint64_t cap, uint64_t *num_nodes,
+                 void *stream) {
+  if (!ctx || !num_nodes) return ALPACC_ERR_ARG;
+  ctx->launches = 0;
+  ctx->err.clear();
+  *num_nodes = 0;
+  if (!ctx->has_parser) {
+    ctx->err = "bundle has no parser";
+    return ALPACC_ERR_NO_PARSER;
+  }
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const uint64_t tcap = n > 0 ? n : 1;
+  uint32_t *kinds = (uint32_t *)ctx->buf(B_TOK_KIND, tcap * 4);
+  uint32_t *starts = (uint32_t *)ctx->buf(B_TOK_START, tcap * 4);
+  uint32_t *ends = (uint32_t *)ctx->buf(B_TOK_END, tcap * 4);
+  if (!kinds || !starts || !ends) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  uint64_t T = 0;
+  int rc = run_lex(ctx, d_in, n, kinds, starts, ends, &T, st);
+  if (rc != ALPACC_ACCEPT) return rc;
+  const int lex_launches = ctx->launches;
+  rc = run_parse_tokens(ctx, kinds, starts, ends, T, d_parent, d_node, d_span, cap, num_nodes, st);
+  ctx->launches += lex_launches;
+  return rc;
+}
+
+int alpacc_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int alpacc_set_device(int device) { return hipSetDevice(device) == hipSuccess ? 0 : -1; }
+
+void *alpacc_device_alloc(uint64_t bytes) {
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+  return p;
+}
+
+void alpacc_device_free(void *ptr) {
+  if (ptr) (void)hipFree(ptr);
+}
+
+int alpacc_copy_to_device(void *dst, const void *src, uint64_t bytes, void *stream) {
+  if (bytes == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  if (hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
+  return hipStreamSynchronize(st) == hipSuccess ? 0 : -1;
+}
+
+int alpacc_copy_to_host(void *dst, const void *src, uint64_t bytes, void *stream) {
+  if (bytes == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+  return hipStreamSynchronize(st) == hipSuccess ? 0 : -1;
+}
+
+int alpacc_device_memset(void *dst, int value, uint64_t bytes, void *stream) {
+  if (bytes == 0) return 0;
+  return hipMemsetAsync(dst, value, bytes, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+void *alpacc_stream_create(void) {
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  return (void *)s;
+}
+
+void alpacc_stream_destroy(void *stream) {
+  if (stream) (void)hipStreamDestroy((hipStream_t)stream);
+}
+
+int alpacc_stream_sync(void *stream) {
+  return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+void *alpacc_event_create(void) {
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return (void *)e;
+}
+
+void alpacc_event_destroy(void *event) {
+  if (event) (void)hipEventDestroy((hipEvent_t)event);
+}
+
+int alpacc_event_record(void *event, void *stream) {
+  return hipEventRecord((hipEvent_t)event, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+float alpacc_event_elapsed_ms(void *start, void *end) {
+  if (hipEventSynchronize((hipEvent_t)end) != hipSuccess) return -1.0f;
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, (hipEvent_t)start, (hipEvent_t)end) != hipSuccess) return -1.0f;
+  return ms;
+}
+
+}  // extern "C"

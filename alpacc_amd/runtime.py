@@ -1,0 +1,350 @@
+"""Host-side mirror of the reference's generated entries, backed by the HIP C ABI.
+
+The reference's Futhark artifact exposes (src/Alpacc/Generator/Futhark/Generator.hs:19-92):
+  * ``lex_int``   : bytes -> (terminal ids, starts, ends), empty arrays when rejected;
+  * ``lex``       : the same with terminal names (lexer.fut:180-191);
+  * ``parse_int`` : bytes -> opt [(parent, #terminal t (i, j) | #production p)];
+  * ``parse``     : the same with names (parser.fut:338-350).
+Here ``lex_int``/``parse_int`` return ``None`` for the reference's ``#none`` (rejection) instead of
+empty arrays, because "accepted with zero tokens" and "rejected" must stay distinguishable.
+
+The library is ``alpacc_amd/libalpacc_hip.so`` (built by ``alpacc_amd.build``); every call runs the
+HIP kernels. There is no CPU fallback: a missing library or GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .gen.tables import Bundle
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libalpacc_hip.so")
+
+ACCEPT, REJECT = 1, 0
+ERRORS = {-1: "bad argument", -2: "HIP error", -3: "capacity", -4: "look-back timeout",
+          -5: "bundle has no parser"}
+NODE_TERMINAL = 0x80000000
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u16p = ctypes.POINTER(ctypes.c_uint16)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+class AlpaccTables(ctypes.Structure):
+    _fields_ = [
+        ("num_endos", ctypes.c_uint32), ("num_classes", ctypes.c_uint32),
+        ("num_states", ctypes.c_uint32), ("init_state", ctypes.c_uint32),
+        ("ignore_terminal", ctypes.c_int32), ("dead_terminal", ctypes.c_uint32),
+        ("byte_class", _u8p), ("endo_step", _u16p), ("endo_comp", _u16p),
+        ("endo_apply", _u8p), ("dfa_step", _u16p), ("state_accept", _u8p),
+        ("q", ctypes.c_uint32), ("k", ctypes.c_uint32), ("num_terminals", ctypes.c_uint32),
+        ("start_terminal", ctypes.c_uint32), ("end_terminal", ctypes.c_uint32),
+        ("empty_terminal", ctypes.c_uint32), ("num_productions", ctypes.c_uint32),
+        ("arities", _i32p), ("prod_to_terminal", _i32p),
+        ("pair_index_len", ctypes.c_uint64), ("pair_index", _u32p),
+        ("num_pair_entries", ctypes.c_uint32), ("pair_entry", _u32p),
+        ("stacks_len", ctypes.c_uint32), ("stacks", _u16p),
+        ("productions_len", ctypes.c_uint32), ("productions", _u16p),
+    ]
+
+
+EXPORTS = ("alpacc_create", "alpacc_create_error", "alpacc_destroy", "alpacc_last_error",
+           "alpacc_lex", "alpacc_parse", "alpacc_parse_tokens", "alpacc_last_launch_count",
+           "alpacc_device_count", "alpacc_set_device", "alpacc_device_alloc", "alpacc_device_free",
+           "alpacc_copy_to_device", "alpacc_copy_to_host", "alpacc_device_memset",
+           "alpacc_stream_create", "alpacc_stream_destroy", "alpacc_stream_sync",
+           "alpacc_event_create", "alpacc_event_destroy", "alpacc_event_record",
+           "alpacc_event_elapsed_ms")
+
+_lib = None
+
+
+def load_library():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"alpacc HIP library missing at {LIB_PATH}; build it with "
+                           "`python __graft_entry__.py build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    u64 = ctypes.c_uint64
+    L.alpacc_create.argtypes = [ctypes.POINTER(AlpaccTables), ctypes.c_int]
+    L.alpacc_create.restype = vp
+    L.alpacc_create_error.restype = ctypes.c_char_p
+    L.alpacc_destroy.argtypes = [vp]
+    L.alpacc_last_error.argtypes = [vp]
+    L.alpacc_last_error.restype = ctypes.c_char_p
+    L.alpacc_last_launch_count.argtypes = [vp]
+    L.alpacc_lex.argtypes = [vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
+    L.alpacc_parse.argtypes = [vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
+    L.alpacc_parse_tokens.argtypes = [vp, vp, vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
+    L.alpacc_device_count.restype = ctypes.c_int
+    L.alpacc_set_device.argtypes = [ctypes.c_int]
+    L.alpacc_device_alloc.argtypes = [u64]
+    L.alpacc_device_alloc.restype = vp
+    L.alpacc_device_free.argtypes = [vp]
+    L.alpacc_copy_to_device.argtypes = [vp, vp, u64, vp]
+    L.alpacc_copy_to_host.argtypes = [vp, vp, u64, vp]
+    L.alpacc_device_memset.argtypes = [vp, ctypes.c_int, u64, vp]
+    L.alpacc_stream_create.restype = vp
+    L.alpacc_stream_destroy.argtypes = [vp]
+    L.alpacc_stream_sync.argtypes = [vp]
+    L.alpacc_event_create.restype = vp
+    L.alpacc_event_destroy.argtypes = [vp]
+    L.alpacc_event_record.argtypes = [vp, vp]
+    L.alpacc_event_elapsed_ms.argtypes = [vp, vp]
+    L.alpacc_event_elapsed_ms.restype = ctypes.c_float
+    _lib = L
+    return L
+
+
+class DeviceBuffer:
+    """A device allocation owned by this library's HIP runtime."""
+
+    def __init__(self, nbytes: int, lib=None):
+        self.lib = lib or load_library()
+        self.nbytes = int(nbytes)
+        p = self.lib.alpacc_device_alloc(max(self.nbytes, 1))
+        if not p:
+            raise MemoryError(f"device allocation of {nbytes} bytes failed")
+        self.ptr = p
+
+    def upload(self, arr: np.ndarray, stream=None, offset: int = 0):
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes + offset > self.nbytes:
+            raise ValueError("upload larger than the buffer")
+        if self.lib.alpacc_copy_to_device(ctypes.c_void_p(self.ptr + offset), arr.ctypes.data,
+                                          arr.nbytes, stream) != 0:
+            raise RuntimeError("host to device copy failed")
+
+    def download(self, dtype, count: int, stream=None, offset: int = 0) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes + offset > self.nbytes:
+            raise ValueError("download larger than the buffer")
+        if self.lib.alpacc_copy_to_host(out.ctypes.data, ctypes.c_void_p(self.ptr + offset),
+                                        out.nbytes, stream) != 0:
+            raise RuntimeError("device to host copy failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.lib.alpacc_device_free(ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    return load_library().alpacc_device_count()
+
+
+def _ptr(arr: np.ndarray, typ):
+    return arr.ctypes.data_as(typ)
+
+
+def fill_tables(bundle: Bundle) -> tuple[AlpaccTables, list]:
+    """The C struct of a bundle plus the numpy arrays it points into (keep them alive)."""
+    t = AlpaccTables()
+    keep = []
+
+    def hold(a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        keep.append(a)
+        return a
+
+    lx = bundle.lexer
+    if lx is not None:
+        t.num_endos = lx.endo.num_endos
+        t.num_classes = lx.num_classes
+        t.num_states = lx.num_states
+        t.init_state = lx.init_state
+        t.ignore_terminal = lx.ignore
+        t.dead_terminal = lx.dead_terminal
+        t.byte_class = _ptr(hold(lx.byte_class, np.uint8), _u8p)
+        t.endo_step = _ptr(hold(lx.endo_step, np.uint16), _u16p)
+        t.endo_comp = _ptr(hold(lx.endo_comp, np.uint16), _u16p)
+        t.endo_apply = _ptr(hold(lx.endo_apply, np.uint8), _u8p)
+        t.dfa_step = _ptr(hold(lx.dfa_step, np.uint16), _u16p)
+        t.state_accept = _ptr(hold(lx.state_info, np.uint8), _u8p)
+    px = bundle.parser
+    if px is not None:
+        t.q, t.k = px.q, px.k
+        t.num_terminals = px.num_terminals
+        t.start_terminal = px.start_terminal
+        t.end_terminal = px.end_terminal
+        t.empty_terminal = px.empty_terminal
+        t.num_productions = len(px.production_names)
+        t.arities = _ptr(hold(px.arities, np.int32), _i32p)
+        t.prod_to_terminal = _ptr(hold(px.production_to_terminal, np.int32), _i32p)
+        t.pair_index_len = px.pair_index.shape[0]
+        t.pair_index = _ptr(hold(px.pair_index, np.uint32), _u32p)
+        t.num_pair_entries = px.pair_entry.shape[0]
+        t.pair_entry = _ptr(hold(px.pair_entry.reshape(-1), np.uint32), _u32p)
+        t.stacks_len = px.stacks.shape[0]
+        t.stacks = _ptr(hold(px.stacks, np.uint16), _u16p)
+        t.productions_len = px.productions.shape[0]
+        t.productions = _ptr(hold(px.productions, np.uint16), _u16p)
+    return t, keep
+
+
+class Context:
+    """A device-resident table bundle (``alpacc_create``) and the four reference entries."""
+
+    def __init__(self, bundle: Bundle, device: int = 0):
+        self.bundle = bundle
+        self.lib = load_library()
+        if self.lib.alpacc_device_count() <= device:
+            raise RuntimeError("no ROCm device available for the alpacc HIP backend")
+        self.device = device
+        self._tables, self._keep = fill_tables(bundle)
+        h = self.lib.alpacc_create(ctypes.byref(self._tables), device)
+        if not h:
+            raise RuntimeError("alpacc_create failed: " + self.lib.alpacc_create_error().decode())
+        self.handle = h
+        self.stream = None          # default stream
+        self._bufs: dict = {}
+
+    def close(self):
+        for b in self._bufs.values():
+            b.free()
+        self._bufs.clear()
+        if getattr(self, "handle", None):
+            self.lib.alpacc_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- helpers ---------------------------------------------------------------------------
+    def _err(self, rc):
+        msg = self.lib.alpacc_last_error(self.handle).decode()
+        return RuntimeError(f"alpacc error {rc} ({ERRORS.get(rc, '?')}): {msg}")
+
+    def buffer(self, name: str, nbytes: int) -> DeviceBuffer:
+        b = self._bufs.get(name)
+        if b is None or b.nbytes < nbytes:
+            if b is not None:
+                b.free()
+            b = DeviceBuffer(max(nbytes, 256), self.lib)
+            self._bufs[name] = b
+        return b
+
+    def upload_doc(self, data) -> tuple[DeviceBuffer, int]:
+        if isinstance(data, (bytes, bytearray, memoryview)):
+            arr = np.frombuffer(bytes(data), dtype=np.uint8)
+        else:
+            arr = np.ascontiguousarray(data, dtype=np.uint8)
+        buf = self.buffer("doc", arr.shape[0])
+        if arr.shape[0]:
+            buf.upload(arr, self.stream)
+        return buf, int(arr.shape[0])
+
+    # -- device-level calls -----------------------------------------------------------------
+    def lex_device(self, d_in: int, n: int, kinds: int, starts: int, ends: int, cap: int,
+                   stream=None) -> tuple[int, int]:
+        """``alpacc_lex`` on device pointers; returns (status, token count)."""
+        nt = ctypes.c_uint64(0)
+        rc = self.lib.alpacc_lex(self.handle, ctypes.c_void_p(d_in), n, ctypes.c_void_p(kinds),
+                                 ctypes.c_void_p(starts), ctypes.c_void_p(ends), cap,
+                                 ctypes.byref(nt), stream)
+        if rc < 0:
+            raise self._err(rc)
+        return rc, nt.value
+
+    def parse_device(self, d_in: int, n: int, parent: int, node: int, span: int, cap: int,
+                     stream=None) -> tuple[int, int]:
+        """``alpacc_parse`` on device pointers; returns (status, node count); status -3 means
+        the capacity was too small and the count is the size needed."""
+        nn = ctypes.c_uint64(0)
+        rc = self.lib.alpacc_parse(self.handle, ctypes.c_void_p(d_in), n, ctypes.c_void_p(parent),
+                                   ctypes.c_void_p(node), ctypes.c_void_p(span), cap,
+                                   ctypes.byref(nn), stream)
+        if rc == -3:
+            return rc, nn.value
+        if rc < 0:
+            raise self._err(rc)
+        return rc, nn.value
+
+    # -- reference entries ------------------------------------------------------------------
+    def lex_int(self, data):
+        """``lex_int``: None when rejected, else (kinds, starts, ends) as uint32 numpy arrays."""
+        d_in, n = self.upload_doc(data)
+        cap = max(n, 1)
+        kb = self.buffer("kinds", 4 * cap)
+        sb = self.buffer("starts", 4 * cap)
+        eb = self.buffer("ends", 4 * cap)
+        rc, nt = self.lex_device(d_in.ptr, n, kb.ptr, sb.ptr, eb.ptr, cap, self.stream)
+        if rc != ACCEPT:
+            return None
+        return (kb.download(np.uint32, nt, self.stream), sb.download(np.uint32, nt, self.stream),
+                eb.download(np.uint32, nt, self.stream))
+
+    def lex(self, data):
+        r = self.lex_int(data)
+        if r is None:
+            return None
+        names = self.bundle.terminal_names
+        return [(names[k], (int(s), int(e))) for k, s, e in zip(*r)]
+
+    def parse_int(self, data, cap: int | None = None):
+        """``parse_int``: None when rejected, else (parent, node, span[N,2]) uint32 numpy arrays.
+
+        ``node`` has bit 31 set for terminal nodes (low bits = terminal id), else a production id.
+        """
+        d_in, n = self.upload_doc(data)
+        if cap is None:
+            cap = max(16, 4 * n + 16)
+        while True:
+            pb = self.buffer("parent", 4 * cap)
+            nb = self.buffer("node", 4 * cap)
+            spb = self.buffer("span", 8 * cap)
+            rc, nn = self.parse_device(d_in.ptr, n, pb.ptr, nb.ptr, spb.ptr, cap, self.stream)
+            if rc == -3:
+                cap = nn
+                continue
+            break
+        if rc != ACCEPT:
+            return None
+        return (pb.download(np.uint32, nn, self.stream), nb.download(np.uint32, nn, self.stream),
+                spb.download(np.uint32, 2 * nn, self.stream).reshape(-1, 2))
+
+    def parse(self, data):
+        r = self.parse_int(data)
+        if r is None:
+            return None
+        tnames = self.bundle.terminal_names
+        pnames = self.bundle.parser.production_names
+        out = []
+        for par, nd, sp in zip(*r):
+            if nd & NODE_TERMINAL:
+                out.append((int(par), ("terminal", tnames[nd & 0x7FFFFFFF], (int(sp[0]), int(sp[1])))))
+            else:
+                out.append((int(par), ("production", pnames[nd])))
+        return out
+
+
+def tree_tuples(r):
+    """(parent, is_terminal, id, start, end) tuples of a parse_int result (the wire-format view)."""
+    if r is None:
+        return None
+    par, node, span = r
+    out = []
+    for p, nd, sp in zip(par.tolist(), node.tolist(), span.tolist()):
+        if nd & NODE_TERMINAL:
+            out.append((p, True, nd & 0x7FFFFFFF, sp[0], sp[1]))
+        else:
+            out.append((p, False, nd, 0, 0))
+    return out

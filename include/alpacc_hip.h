@@ -1,0 +1,129 @@
+/*
+ * alpacc_hip.h - C ABI of the MI355X (gfx950) lexer/parser backend for alpacc grammars.
+ *
+ * The reference exposes its runtime as the entries of a generated Futhark program
+ * (src/Alpacc/Generator/Futhark/Generator.hs:19-92):
+ *   lex_int   : []u8 -> ([]terminal_int, []i64, []i64), empty arrays on rejection
+ *   parse_int : []u8 -> opt [](i64, node terminal_int production_int)
+ * built from the table bundle of src/Alpacc/Generator/Analyzer.hs:33-82 and the generic runtime
+ * futhark/lexer.fut + futhark/parser.fut. This header replaces those entries with plain C calls:
+ * tables go in once (alpacc_create), documents are device buffers, outputs are caller-allocated
+ * device buffers, and rejection is an explicit return value instead of an empty/none result.
+ *
+ * Return codes: 1 = accepted, 0 = rejected (the reference's #none), < 0 = error.
+ * Device pointers are HIP device (or host-coherent) addresses; `stream` is a hipStream_t (or NULL
+ * for the default stream). Calls block until their results are known. One context may be used by
+ * one host thread at a time.
+ */
+#ifndef ALPACC_HIP_H
+#define ALPACC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALPACC_ACCEPT 1
+#define ALPACC_REJECT 0
+#define ALPACC_ERR_ARG (-1)
+#define ALPACC_ERR_HIP (-2)
+#define ALPACC_ERR_CAPACITY (-3)
+#define ALPACC_ERR_TIMEOUT (-4)
+#define ALPACC_ERR_NO_PARSER (-5)
+
+/* Node word of a parse tree: bit 31 set = terminal node (low bits = terminal id), else a
+ * production node (low bits = production id). */
+#define ALPACC_NODE_TERMINAL 0x80000000u
+
+/* The generated tables of one grammar (the reference's Analyzer record, Analyzer.hs:33-82),
+ * in the device layouts described in alpacc_amd/gen/tables.py. Host pointers; alpacc_create
+ * copies them. Parser fields may be zero/NULL for a lexer-only bundle. */
+typedef struct alpacc_tables {
+  /* lexer (Analyzer.hs:33-41, futhark/lexer.fut:8-27) */
+  uint32_t num_endos;        /* e: endomorphisms (identity = 0, dead = 1) */
+  uint32_t num_classes;      /* C: byte classes */
+  uint32_t num_states;       /* m + 1: lexer DFA states plus the dead state 0 */
+  uint32_t init_state;       /* DFA initial state (in 1..m) */
+  int32_t ignore_terminal;   /* -1 when the grammar has no ignore token */
+  uint32_t dead_terminal;
+  const uint8_t *byte_class;  /* [256] */
+  const uint16_t *endo_step;  /* [e*C]  endo a, then one byte of class c */
+  const uint16_t *endo_comp;  /* [e*e]  endo a then endo b, row a */
+  const uint8_t *endo_apply;  /* [e*(m+1)] endo a applied to state s */
+  const uint16_t *dfa_step;   /* [(m+1)*C] next | produce<<8 | label(next)<<9 */
+  const uint8_t *state_accept;/* [m+1] */
+  /* parser (Analyzer.hs:52-66, futhark/parser.fut:9-33) */
+  uint32_t q, k;
+  uint32_t num_terminals;    /* terminal symbol ids are [0, num_terminals) */
+  uint32_t start_terminal, end_terminal, empty_terminal;
+  uint32_t num_productions;
+  const int32_t *arities;          /* [P] */
+  const int32_t *prod_to_terminal; /* [P], -1 = not a terminal production */
+  uint64_t pair_index_len;         /* num_terminals^(q+k) */
+  const uint32_t *pair_index;      /* dense pair table: entry id or 0xFFFFFFFF */
+  uint32_t num_pair_entries;
+  const uint32_t *pair_entry;      /* [E*4] stack start/end, production start/end */
+  uint32_t stacks_len;
+  const uint16_t *stacks;          /* brackets: symbol | 0x8000 for left */
+  uint32_t productions_len;
+  const uint16_t *productions;
+} alpacc_tables;
+
+typedef struct alpacc_ctx alpacc_ctx;
+
+/* Upload a table bundle to `device`; NULL on failure (see alpacc_create_error). */
+alpacc_ctx *alpacc_create(const alpacc_tables *t, int device);
+const char *alpacc_create_error(void);
+void alpacc_destroy(alpacc_ctx *ctx);
+const char *alpacc_last_error(alpacc_ctx *ctx);
+
+/* lex_int (futhark/lexer.fut:174-178; entry Generator.hs:70-75), unchunked semantics.
+ * in: device bytes [n]; outputs: device u32 [cap] each (cap >= n suffices). On return
+ * *num_tokens holds the token count (also when rejected, in which case outputs are undefined). */
+int alpacc_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_kinds,
+               uint32_t *d_starts, uint32_t *d_ends, uint64_t cap, uint64_t *num_tokens,
+               void *stream);
+
+/* parse_int of a lexer+parser artifact (Generator.hs:35-44: lex_int then parser.parse_int,
+ * futhark/parser.fut:316-336). Outputs device u32 arrays: parent[cap], node[cap], span[2*cap]
+ * (start, end; zeros for production nodes). If cap is too small the call returns
+ * ALPACC_ERR_CAPACITY with *num_nodes set to the size needed. */
+int alpacc_parse(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_parent,
+                 uint32_t *d_node, uint32_t *d_span, uint64_t cap, uint64_t *num_nodes,
+                 void *stream);
+
+/* Parser stage alone on a device token stream (parse_int of futhark/parser.fut:334-336):
+ * kinds/starts/ends are the lexer outputs of T tokens. */
+int alpacc_parse_tokens(alpacc_ctx *ctx, const uint32_t *d_kinds, const uint32_t *d_starts,
+                        const uint32_t *d_ends, uint64_t num_tokens, uint32_t *d_parent,
+                        uint32_t *d_node, uint32_t *d_span, uint64_t cap, uint64_t *num_nodes,
+                        void *stream);
+
+/* Number of kernels the last call on `ctx` launched. */
+int alpacc_last_launch_count(alpacc_ctx *ctx);
+
+/* Device plumbing for hosts that have no HIP binding of their own (the Python host, ctypes
+ * callers, FFI callers). They share this library's HIP runtime instance, which a caller's
+ * separately loaded HIP runtime would not. All return 0 / non-NULL on success. */
+int alpacc_device_count(void);
+int alpacc_set_device(int device);
+void *alpacc_device_alloc(uint64_t bytes);
+void alpacc_device_free(void *ptr);
+int alpacc_copy_to_device(void *dst, const void *src, uint64_t bytes, void *stream);
+int alpacc_copy_to_host(void *dst, const void *src, uint64_t bytes, void *stream);
+int alpacc_device_memset(void *dst, int value, uint64_t bytes, void *stream);
+void *alpacc_stream_create(void);
+void alpacc_stream_destroy(void *stream);
+int alpacc_stream_sync(void *stream);
+void *alpacc_event_create(void);
+void alpacc_event_destroy(void *event);
+int alpacc_event_record(void *event, void *stream);
+/* milliseconds between two recorded events (waits for `end`); negative on error */
+float alpacc_event_elapsed_ms(void *start, void *end);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALPACC_HIP_H */
