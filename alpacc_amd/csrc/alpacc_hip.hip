@@ -670,6 +670,25 @@ struct alpacc_ctx {
   // scratch (grow-only)
   std::vector<std::pair<void *, size_t>> scratch;
   uint32_t *h_small = nullptr;  // pinned host words for results
+  // profiling: events around the lexer kernel and around the parser stage
+  bool prof = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool ev_lex = false, ev_parse = false;
+  double prof_lex_ms = 0.0, prof_parse_ms = 0.0;
+  uint64_t prof_lex_calls = 0, prof_parse_calls = 0;
+
+  void prof_collect() {
+    float ms = 0.f;
+    if (ev_lex && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) {
+      prof_lex_ms += ms;
+      prof_lex_calls++;
+    }
+    if (ev_parse && hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) {
+      prof_parse_ms += ms;
+      prof_parse_calls++;
+    }
+    ev_lex = ev_parse = false;
+  }
 
   void *buf(int id, size_t bytes) {
     if ((int)scratch.size() <= id) scratch.resize(id + 1, {nullptr, 0});
@@ -848,10 +867,15 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   const uint64_t waves_per_block = LEX_BLOCK / WAVE;
   uint64_t blocks = (tiles + waves_per_block - 1) / waves_per_block;
   blocks = std::min<uint64_t>(blocks, (uint64_t)std::max(1, ctx->cus) * 2);
+  if (ctx->prof) CK(hipEventRecord(ctx->ev[0], st));
   if (ctx->comp_in_lds)
     lex_kernel<true><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
   else
     lex_kernel<false><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
+  if (ctx->prof) {
+    CK(hipEventRecord(ctx->ev[1], st));
+    ctx->ev_lex = true;
+  }
   ctx->launches += 4;
   CK(hipGetLastError());
   CK(hipMemcpyAsync(ctx->h_small, small + 4, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -897,6 +921,10 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *sta
   a.arity = ctx->d_arity;
   a.p2t = ctx->d_p2t;
   a.flags = flags;
+  if (ctx->prof) {
+    CK(hipEventRecord(ctx->ev[2], st));
+    ctx->ev_parse = true;
+  }
   clear_u32<<<1, 64, 0, st>>>(flags, 4);
   parse_keys<<<grid_for(P, 256), 256, 0, st>>>(a, ent, lens);
   ctx->launches += 2;
@@ -972,6 +1000,7 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *sta
                                                 span);
   ctx->launches++;
   CK(hipGetLastError());
+  if (ctx->prof) CK(hipEventRecord(ctx->ev[3], st));
   CK(hipStreamSynchronize(st));
   return ALPACC_ACCEPT;
 }
@@ -1080,12 +1109,38 @@ void alpacc_destroy(alpacc_ctx *ctx) {
   for (void *p : tabs)
     if (p) (void)hipFree(p);
   if (ctx->h_small) (void)hipHostFree(ctx->h_small);
+  for (auto &e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
   delete ctx;
 }
 
 const char *alpacc_last_error(alpacc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int alpacc_last_launch_count(alpacc_ctx *ctx) { return ctx ? ctx->launches : 0; }
+
+int alpacc_profile_enable(alpacc_ctx *ctx, int on) {
+  if (!ctx) return ALPACC_ERR_ARG;
+  if (on && !ctx->ev[0]) {
+    for (auto &e : ctx->ev)
+      if (hipEventCreate(&e) != hipSuccess) return ALPACC_ERR_HIP;
+  }
+  ctx->prof = on != 0;
+  ctx->prof_lex_ms = ctx->prof_parse_ms = 0.0;
+  ctx->prof_lex_calls = ctx->prof_parse_calls = 0;
+  return 0;
+}
+
+int alpacc_profile_read(alpacc_ctx *ctx, double *lex_ms, uint64_t *lex_calls, double *parse_ms,
+                        uint64_t *parse_calls) {
+  if (!ctx) return ALPACC_ERR_ARG;
+  if (lex_ms) *lex_ms = ctx->prof_lex_ms;
+  if (lex_calls) *lex_calls = ctx->prof_lex_calls;
+  if (parse_ms) *parse_ms = ctx->prof_parse_ms;
+  if (parse_calls) *parse_calls = ctx->prof_parse_calls;
+  ctx->prof_lex_ms = ctx->prof_parse_ms = 0.0;
+  ctx->prof_lex_calls = ctx->prof_parse_calls = 0;
+  return 0;
+}
 
 int alpacc_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_kinds,
                uint32_t *d_starts, uint32_t *d_ends, uint64_t cap, uint64_t *num_tokens,
@@ -1098,7 +1153,10 @@ int alpacc_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_kin
   ctx->launches = 0;
   ctx->err.clear();
   (void)hipSetDevice(ctx->device);
-  return run_lex(ctx, d_in, n, d_kinds, d_starts, d_ends, num_tokens, (hipStream_t)stream);
+  const int rc = run_lex(ctx, d_in, n, d_kinds, d_starts, d_ends, num_tokens, (hipStream_t)stream);
+  ctx->ev_parse = false;
+  if (ctx->prof) ctx->prof_collect();
+  return rc;
 }
 
 int alpacc_parse_tokens(alpacc_ctx *ctx, const uint32_t *d_kinds, const uint32_t *d_starts,
@@ -1109,8 +1167,12 @@ int alpacc_parse_tokens(alpacc_ctx *ctx, const uint32_t *d_kinds, const uint32_t
   ctx->launches = 0;
   ctx->err.clear();
   (void)hipSetDevice(ctx->device);
-  return run_parse_tokens(ctx, d_kinds, d_starts, d_ends, num_tokens, d_parent, d_node, d_span,
-                          cap, num_nodes, (hipStream_t)stream);
+  const int rc = run_parse_tokens(ctx, d_kinds, d_starts, d_ends, num_tokens, d_parent, d_node,
+                                  d_span, cap, num_nodes, (hipStream_t)stream);
+  if (rc != ALPACC_ACCEPT) ctx->ev_parse = false;
+  ctx->ev_lex = false;
+  if (ctx->prof) ctx->prof_collect();
+  return rc;
 }
 
 int alpacc_parse(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_parent,
@@ -1136,10 +1198,16 @@ int alpacc_parse(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_p
   }
   uint64_t T = 0;
   int rc = run_lex(ctx, d_in, n, kinds, starts, ends, &T, st);
-  if (rc != ALPACC_ACCEPT) return rc;
+  if (rc != ALPACC_ACCEPT) {
+    ctx->ev_parse = false;
+    if (ctx->prof) ctx->prof_collect();
+    return rc;
+  }
   const int lex_launches = ctx->launches;
   rc = run_parse_tokens(ctx, kinds, starts, ends, T, d_parent, d_node, d_span, cap, num_nodes, st);
   ctx->launches += lex_launches;
+  if (rc != ALPACC_ACCEPT) ctx->ev_parse = false;
+  if (ctx->prof) ctx->prof_collect();
   return rc;
 }
 
