@@ -37,6 +37,9 @@ constexpr uint32_t ST_INC = 2u << 30;
 constexpr uint64_t CT_AGG = 1ull << 62;
 constexpr uint64_t CT_INC = 2ull << 62;
 constexpr uint32_t SPIN_LIMIT = 1u << 22;        // bounded waits: a stuck look-back ends the kernel
+constexpr int LEX_CT_WAVES = LEX_BLOCK / WAVE;   // wave tiles per count tile (one block)
+constexpr int LEX_SCAN_BLOCK = 1024;             // wave tiles per K2a scan block
+constexpr int LEX_MAX_DFA = 8192;                // max (m+1)*C entries of the DFA table in LDS
 
 // ----------------------------------------------------------------------------------------------
 // small device helpers
@@ -79,11 +82,16 @@ struct LexArgs {
   const uint8_t *state_accept;
   uint32_t e, C, m1, init;
   int32_t ignore;
-  uint32_t num_tiles;
+  uint32_t num_tiles;         // wave tiles of LEX_TILE bytes
+  uint32_t num_ctiles;        // count tiles of LEX_CT_WAVES wave tiles
   uint32_t aligned;           // input pointer is 16-byte aligned
-  uint32_t *tile_counter;     // [1]
-  uint32_t *st_state;         // [num_tiles]
-  uint64_t *st_count;         // [num_tiles]
+  uint16_t *lane_excl;        // [num_tiles*64] exclusive endomorphism of each lane in its wave tile
+  uint16_t *tile_agg;         // [num_tiles] endomorphism of each wave tile
+  uint16_t *tile_excl;        // [num_tiles] exclusive endomorphism within its scan block
+  uint16_t *block_agg;        // [num scan blocks]
+  uint8_t *tile_state;        // [num_tiles] lexer DFA state at the start of each wave tile
+  uint32_t *tile_counter;     // [1] dynamic count-tile index
+  uint64_t *st_count;         // [num_ctiles] look-back words of the count scan
   uint32_t *result;           // [0] accept, [1] tokens, [2] timeout flag
 };
 
@@ -92,43 +100,180 @@ __device__ __forceinline__ uint32_t comp_lookup(const uint16_t *tab, uint32_t e,
   return tab[a * e + b];
 }
 
-// Exclusive prefix endomorphism of `tile` from the state look-back window (decoupled look-back,
-// combined strictly in tile order because composition is not commutative).
-__device__ uint32_t lookback_state(const LexArgs &p, const uint16_t *comp_tab, uint32_t tile,
-                                   int lane) {
-  uint32_t prefix = 0;  // identity
-  int64_t j = (int64_t)tile - 1;
-  uint32_t spins = 0;
-  while (true) {
-    const int64_t idx = j - lane;
-    const uint32_t w = idx >= 0 ? ld_relaxed(p.st_state + idx) : (ST_INC | 0u);
-    const uint32_t flag = w >> 30;
-    const uint64_t incm = __ballot(flag == 2);
-    const uint64_t zerom = __ballot(flag == 0);
-    const uint64_t upto = incm ? (((incm & (~incm + 1)) << 1) - 1) : ~0ull;
-    if (zerom & upto) {
-      if (++spins > SPIN_LIMIT || ld_relaxed(p.result + 2) != 0) {
-        if (lane == 0) st_relaxed(p.result + 2, 1u);
-        return 0;
+#define LEX_CLS(j) ((cls[(j) >> 2] >> (8 * ((j)&3))) & 0xFFu)
+
+// bytes of one lane -> byte classes, four per register
+__device__ __forceinline__ void load_classes(const LexArgs &p, const uint8_t *s_cls, uint64_t base,
+                                             int valid, uint32_t (&cls)[LEX_IPT / 4]) {
+  if (valid == LEX_IPT && p.aligned) {
+    const uint4 a = *reinterpret_cast<const uint4 *>(p.in + base);
+    const uint4 b = *reinterpret_cast<const uint4 *>(p.in + base + 16);
+    const uint32_t wds[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const uint32_t x = wds[q];
+      cls[q] = (uint32_t)s_cls[x & 0xFF] | ((uint32_t)s_cls[(x >> 8) & 0xFF] << 8) |
+               ((uint32_t)s_cls[(x >> 16) & 0xFF] << 16) | ((uint32_t)s_cls[x >> 24] << 24);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = q * 4 + r;
+        if (j < valid) v |= (uint32_t)s_cls[p.in[base + j]] << (8 * r);
       }
-      __builtin_amdgcn_s_sleep(1);
-      continue;
+      cls[q] = v;
     }
-    uint32_t v = ((upto >> lane) & 1ull) ? (w & 0xFFFFu) : 0u;
-    const int kmax = incm ? (__ffsll((long long)incm) - 1) : 63;
-    // order-preserving tree: lane l (later tile) absorbs lane l+d (earlier tiles) first
-    for (int d = 1; d <= kmax; d <<= 1) {
-      const uint32_t other = __shfl_down(v, d, WAVE);
-      if ((lane & (2 * d - 1)) == 0 && lane + d <= kmax) v = comp_lookup(comp_tab, p.e, other, v);
-    }
-    const uint32_t r = __shfl(v, 0, WAVE);
-    prefix = comp_lookup(comp_tab, p.e, r, prefix);
-    if (incm) return prefix;
-    j -= WAVE;
   }
 }
 
-// Exclusive (token count, last token start) of `tile`; both combine commutatively.
+// K1: per wave tile, the endomorphism of every lane's bytes (serial walk through the
+// (endo x class) table in LDS) and their wave-exclusive composition (Blelloch up/down sweep with
+// the (endo x endo) table, 126 lookups per wave). Persistent, grid-stride over wave tiles.
+template <bool COMP_IN_LDS>
+__global__ void __launch_bounds__(LEX_BLOCK) lex_reduce(LexArgs p) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const uint32_t e = p.e, C = p.C;
+  uint16_t *s_step = reinterpret_cast<uint16_t *>(smem);
+  uint8_t *s_cls = reinterpret_cast<uint8_t *>(s_step + e * C);
+  uint16_t *s_comp = reinterpret_cast<uint16_t *>(s_cls + 256);
+  for (uint32_t i = threadIdx.x; i < e * C; i += blockDim.x) s_step[i] = p.endo_step[i];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_cls[i] = p.byte_class[i];
+  if (COMP_IN_LDS)
+    for (uint32_t i = threadIdx.x; i < e * e; i += blockDim.x) s_comp[i] = p.endo_comp[i];
+  __syncthreads();
+  const uint16_t *comp_tab = COMP_IN_LDS ? s_comp : p.endo_comp;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const uint32_t wpb = blockDim.x / WAVE;
+  const uint64_t n = p.n;
+  for (uint32_t tile = blockIdx.x * wpb + threadIdx.x / WAVE; tile < p.num_tiles;
+       tile += gridDim.x * wpb) {
+    const uint64_t base = (uint64_t)tile * LEX_TILE + (uint64_t)lane * LEX_IPT;
+    const int valid = base >= n ? 0 : (int)min((uint64_t)LEX_IPT, n - base);
+    uint32_t cls[LEX_IPT / 4];
+    load_classes(p, s_cls, base, valid, cls);
+    uint32_t v = 0;
+#pragma unroll
+    for (int j = 0; j < LEX_IPT; ++j)
+      if (j < valid) v = s_step[v * C + LEX_CLS(j)];
+    // up-sweep: lane l with (l+1) % 2d == 0 holds the composition of lanes l-2d+1 .. l
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+      const uint32_t left = __shfl(v, (lane - d) & (WAVE - 1), WAVE);
+      if (((lane + 1) & (2 * d - 1)) == 0) v = comp_lookup(comp_tab, e, left, v);
+    }
+    const uint32_t total = __shfl(v, WAVE - 1, WAVE);
+    if (lane == WAVE - 1) v = 0;
+    // down-sweep: right node r gets prefix(r) then sum(left subtree); left node gets prefix(r)
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) {
+      const uint32_t left = __shfl(v, (lane - d) & (WAVE - 1), WAVE);
+      const uint32_t right = __shfl(v, (lane + d) & (WAVE - 1), WAVE);
+      if (((lane + 1) & (2 * d - 1)) == 0)
+        v = comp_lookup(comp_tab, e, v, left);
+      else if (((lane + 1 + d) & (2 * d - 1)) == 0)
+        v = right;
+    }
+    p.lane_excl[(uint64_t)tile * WAVE + lane] = (uint16_t)v;
+    if (lane == 0) p.tile_agg[tile] = (uint16_t)total;
+  }
+}
+
+// K2a: exclusive composition of the wave-tile endomorphisms inside blocks of LEX_SCAN_BLOCK tiles
+// (Blelloch in LDS), the block totals to block_agg.
+__global__ void __launch_bounds__(LEX_SCAN_BLOCK) lex_scan_tiles(LexArgs p) {
+  __shared__ uint16_t s[LEX_SCAN_BLOCK];
+  const uint32_t t = blockIdx.x * LEX_SCAN_BLOCK + threadIdx.x;
+  s[threadIdx.x] = t < p.num_tiles ? p.tile_agg[t] : 0;
+  __syncthreads();
+  const uint32_t i = threadIdx.x;
+  for (uint32_t d = 1; d < LEX_SCAN_BLOCK; d <<= 1) {
+    uint32_t nv = 0;
+    const bool act = ((i + 1) & (2 * d - 1)) == 0;
+    if (act) nv = comp_lookup(p.endo_comp, p.e, s[i - d], s[i]);
+    __syncthreads();
+    if (act) s[i] = (uint16_t)nv;
+    __syncthreads();
+  }
+  if (i == LEX_SCAN_BLOCK - 1) {
+    p.block_agg[blockIdx.x] = s[i];
+    s[i] = 0;
+  }
+  __syncthreads();
+  for (uint32_t d = LEX_SCAN_BLOCK / 2; d >= 1; d >>= 1) {
+    const bool right = ((i + 1) & (2 * d - 1)) == 0;
+    uint32_t l = 0, r = 0;
+    if (right) {
+      l = s[i - d];
+      r = s[i];
+    }
+    __syncthreads();
+    if (right) {
+      s[i - d] = (uint16_t)r;
+      s[i] = (uint16_t)comp_lookup(p.endo_comp, p.e, r, l);
+    }
+    __syncthreads();
+  }
+  if (t < p.num_tiles) p.tile_excl[t] = s[i];
+}
+
+// K2b: one block, exclusive composition of the block totals in place (chunks with a carry).
+__global__ void __launch_bounds__(LEX_SCAN_BLOCK) lex_scan_blocks(LexArgs p, uint32_t nblocks) {
+  __shared__ uint16_t s[LEX_SCAN_BLOCK];
+  __shared__ uint16_t carry_s;
+  const uint32_t i = threadIdx.x;
+  if (i == 0) carry_s = 0;
+  for (uint32_t off = 0; off < nblocks; off += LEX_SCAN_BLOCK) {
+    s[i] = off + i < nblocks ? p.block_agg[off + i] : 0;
+    __syncthreads();
+    for (uint32_t d = 1; d < LEX_SCAN_BLOCK; d <<= 1) {
+      uint32_t nv = 0;
+      const bool act = ((i + 1) & (2 * d - 1)) == 0;
+      if (act) nv = comp_lookup(p.endo_comp, p.e, s[i - d], s[i]);
+      __syncthreads();
+      if (act) s[i] = (uint16_t)nv;
+      __syncthreads();
+    }
+    uint32_t chunk_total = 0;
+    if (i == LEX_SCAN_BLOCK - 1) {
+      chunk_total = s[i];
+      s[i] = 0;
+    }
+    __syncthreads();
+    for (uint32_t d = LEX_SCAN_BLOCK / 2; d >= 1; d >>= 1) {
+      const bool right = ((i + 1) & (2 * d - 1)) == 0;
+      uint32_t l = 0, r = 0;
+      if (right) {
+        l = s[i - d];
+        r = s[i];
+      }
+      __syncthreads();
+      if (right) {
+        s[i - d] = (uint16_t)r;
+        s[i] = (uint16_t)comp_lookup(p.endo_comp, p.e, r, l);
+      }
+      __syncthreads();
+    }
+    const uint32_t carry = carry_s;
+    if (off + i < nblocks) p.block_agg[off + i] = (uint16_t)comp_lookup(p.endo_comp, p.e, carry, s[i]);
+    __syncthreads();
+    if (i == LEX_SCAN_BLOCK - 1) carry_s = (uint16_t)comp_lookup(p.endo_comp, p.e, carry, chunk_total);
+    __syncthreads();
+  }
+}
+
+// K2c: DFA state at the start of every wave tile.
+__global__ void lex_tile_states(LexArgs p) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= p.num_tiles) return;
+  const uint32_t pre = comp_lookup(p.endo_comp, p.e, p.block_agg[t / LEX_SCAN_BLOCK], p.tile_excl[t]);
+  p.tile_state[t] = p.endo_apply[pre * p.m1 + p.init];
+}
+
+// Exclusive (token count, last token start) of a count tile: decoupled look-back; the two
+// components combine by + and max, so the window reduces with plain wave reductions.
 __device__ void lookback_count(const LexArgs &p, uint32_t tile, int lane, uint32_t &cnt,
                                int32_t &last) {
   uint32_t csum = 0;
@@ -168,191 +313,170 @@ __device__ __forceinline__ uint64_t pack_count(uint32_t cnt, int32_t last, uint6
   return flag | ((uint64_t)(uint32_t)(last + 1) << 31) | (uint64_t)cnt;
 }
 
-#define LEX_CLS(j) ((cls[(j) >> 2] >> (8 * ((j)&3))) & 0xFFu)
-
-template <bool COMP_IN_LDS>
-__global__ void __launch_bounds__(LEX_BLOCK) lex_kernel(LexArgs p) {
-  extern __shared__ __align__(16) uint8_t smem[];
-  const uint32_t e = p.e, C = p.C, m1 = p.m1;
-  uint16_t *s_step = reinterpret_cast<uint16_t *>(smem);
-  uint16_t *s_dfa = s_step + e * C;
-  uint8_t *s_cls = reinterpret_cast<uint8_t *>(s_dfa + m1 * C);
-  uint16_t *s_comp = reinterpret_cast<uint16_t *>(s_cls + 256);
-  for (uint32_t i = threadIdx.x; i < e * C; i += blockDim.x) s_step[i] = p.endo_step[i];
-  for (uint32_t i = threadIdx.x; i < m1 * C; i += blockDim.x) s_dfa[i] = p.dfa_step[i];
+// K3: walk the lexer DFA from the known lane start states, count token ends / find token starts,
+// scan them across the count tile (16 wave tiles) and across count tiles (decoupled look-back),
+// then walk again and write (kind, start, end) of every token.
+__global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
+  __shared__ uint16_t s_dfa[LEX_MAX_DFA];
+  __shared__ uint8_t s_cls[256];
+  __shared__ uint32_t s_wcnt[LEX_CT_WAVES];
+  __shared__ int32_t s_wlast[LEX_CT_WAVES];
+  __shared__ uint32_t s_ctile;
+  __shared__ uint32_t s_cpre;
+  __shared__ int32_t s_lpre;
+  const uint32_t C = p.C;
+  for (uint32_t i = threadIdx.x; i < p.m1 * C; i += blockDim.x) s_dfa[i] = p.dfa_step[i];
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_cls[i] = p.byte_class[i];
-  if (COMP_IN_LDS)
-    for (uint32_t i = threadIdx.x; i < e * e; i += blockDim.x) s_comp[i] = p.endo_comp[i];
+  if (threadIdx.x == 0) s_ctile = atomicAdd(p.tile_counter, 1u);
   __syncthreads();
-  const uint16_t *comp_tab = COMP_IN_LDS ? s_comp : p.endo_comp;
+  const uint32_t ct = s_ctile;
   const int lane = threadIdx.x & (WAVE - 1);
+  const int w = threadIdx.x / WAVE;
+  const uint32_t tile = ct * LEX_CT_WAVES + w;
   const uint64_t n = p.n;
   const uint32_t ign = p.ignore < 0 ? 0xFFFFFFFFu : (uint32_t)p.ignore;
+  const uint64_t tbase = (uint64_t)tile * LEX_TILE;
+  const uint64_t base = tbase + (uint64_t)lane * LEX_IPT;
+  const int valid = (tile >= p.num_tiles || base >= n) ? 0 : (int)min((uint64_t)LEX_IPT, n - base);
+  uint32_t cls[LEX_IPT / 4];
+  load_classes(p, s_cls, base, valid, cls);
+  uint32_t s_lane = 0;
+  if (tile < p.num_tiles) {
+    const uint32_t s_tile = p.tile_state[tile];
+    const uint32_t ex = p.lane_excl[(uint64_t)tile * WAVE + lane];
+    s_lane = p.endo_apply[ex * p.m1 + s_tile];
+  }
+  uint32_t c_next = __shfl_down(LEX_CLS(0), 1, WAVE);
+  if (lane == WAVE - 1) {
+    const uint64_t nb = tbase + LEX_TILE;
+    c_next = nb < n ? s_cls[p.in[nb]] : 0u;
+  }
 
-  while (true) {
-    uint32_t tile = 0;
-    if (lane == 0) tile = atomicAdd(p.tile_counter, 1u);
-    tile = __shfl(tile, 0, WAVE);
-    if (tile >= p.num_tiles) break;
-    const uint64_t tbase = (uint64_t)tile * LEX_TILE;
-    const uint64_t base = tbase + (uint64_t)lane * LEX_IPT;
-    const int valid = base >= n ? 0 : (int)min((uint64_t)LEX_IPT, n - base);
-
-    // ---- bytes -> classes (packed 4 per register)
-    uint32_t cls[LEX_IPT / 4];
-    if (valid == LEX_IPT && p.aligned) {
-      const uint4 a = *reinterpret_cast<const uint4 *>(p.in + base);
-      const uint4 b = *reinterpret_cast<const uint4 *>(p.in + base + 16);
-      const uint32_t wds[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  // ---- pass 1: count token ends and find the last token start
+  uint32_t cnt = 0;
+  int32_t last = -1;
+  uint32_t s_end = s_lane;
+  uint32_t lab_end = 0;
+  if (valid > 0) {
+    uint32_t ent = s_dfa[s_lane * C + LEX_CLS(0)];
+    uint32_t s = ent & 0xFFu;
+    uint32_t lab = ent >> 9;
+    if (((ent >> 8) & 1u) && lab != ign) last = (int32_t)base;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const uint32_t x = wds[q];
-        cls[q] = (uint32_t)s_cls[x & 0xFF] | ((uint32_t)s_cls[(x >> 8) & 0xFF] << 8) |
-                 ((uint32_t)s_cls[(x >> 16) & 0xFF] << 16) | ((uint32_t)s_cls[x >> 24] << 24);
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        uint32_t v = 0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int j = q * 4 + r;
-          if (j < valid) v |= (uint32_t)s_cls[p.in[base + j]] << (8 * r);
-        }
-        cls[q] = v;
-      }
-    }
-
-    // ---- pass 1: endomorphism of this lane's bytes
-    uint32_t a = 0;
-#pragma unroll
-    for (int j = 0; j < LEX_IPT; ++j)
-      if (j < valid) a = s_step[a * C + LEX_CLS(j)];
-
-    // ---- wave64 inclusive scan of endomorphisms (earlier lane first)
-    uint32_t incl = a;
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-      const uint32_t other = __shfl_up(incl, d, WAVE);
-      if (lane >= d) incl = comp_lookup(comp_tab, e, other, incl);
-    }
-    const uint32_t agg = __shfl(incl, WAVE - 1, WAVE);
-    if (lane == 0) st_relaxed(p.st_state + tile, (tile == 0 ? ST_INC : ST_AGG) | agg);
-    uint32_t prefix = 0;
-    if (tile > 0) {
-      prefix = lookback_state(p, comp_tab, tile, lane);
-      if (lane == 0) st_relaxed(p.st_state + tile, ST_INC | comp_lookup(comp_tab, e, prefix, agg));
-    }
-
-    // ---- lane start states
-    uint32_t excl = __shfl_up(incl, 1, WAVE);
-    if (lane == 0) excl = 0;
-    const uint32_t s_tile = p.endo_apply[prefix * m1 + p.init];
-    const uint32_t s_lane = p.endo_apply[excl * m1 + s_tile];
-    uint32_t c_next = __shfl_down(LEX_CLS(0), 1, WAVE);
-    if (lane == WAVE - 1) {
-      const uint64_t nb = tbase + LEX_TILE;
-      c_next = nb < n ? s_cls[p.in[nb]] : 0u;
-    }
-
-    // ---- pass 2: count token ends and find the last token start
-    uint32_t cnt = 0;
-    int32_t last = -1;
-    uint32_t s_end = s_lane;
-    uint32_t lab_end = 0;
-    if (valid > 0) {
-      uint32_t ent = s_dfa[s_lane * C + LEX_CLS(0)];
-      uint32_t s = ent & 0xFFu;
-      uint32_t lab = ent >> 9;
-      if (((ent >> 8) & 1u) && lab != ign) last = (int32_t)base;
-#pragma unroll
-      for (int j = 0; j < LEX_IPT; ++j) {
-        if (j < valid) {
-          const bool inner = j + 1 < valid;
-          const bool has_next = inner || (base + j + 1 < n);
-          const uint32_t cn = inner ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
-          const uint32_t ent2 = has_next ? s_dfa[s * C + cn] : 0u;
-          const bool prod2 = (ent2 >> 8) & 1u;
-          const uint32_t lab2 = ent2 >> 9;
-          cnt += (has_next && prod2 && lab != ign) ? 1u : 0u;
-          if (inner) {
-            if (prod2 && lab2 != ign) last = (int32_t)(base + j + 1);
-            s = ent2 & 0xFFu;
-            lab = lab2;
-          }
+    for (int j = 0; j < LEX_IPT; ++j) {
+      if (j < valid) {
+        const bool inner = j + 1 < valid;
+        const bool has_next = inner || (base + j + 1 < n);
+        const uint32_t cn = inner ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
+        const uint32_t ent2 = has_next ? s_dfa[s * C + cn] : 0u;
+        const bool prod2 = (ent2 >> 8) & 1u;
+        const uint32_t lab2 = ent2 >> 9;
+        cnt += (has_next && prod2 && lab != ign) ? 1u : 0u;
+        if (inner) {
+          if (prod2 && lab2 != ign) last = (int32_t)(base + j + 1);
+          s = ent2 & 0xFFu;
+          lab = lab2;
         }
       }
-      s_end = s;
-      lab_end = lab;
     }
+    s_end = s;
+    lab_end = lab;
+  }
 
-    // ---- wave scan of (count, last start)
-    uint32_t cincl = cnt;
-    int32_t lincl = last;
+  // ---- scans: wave, count tile, across count tiles
+  uint32_t cincl = cnt;
+  int32_t lincl = last;
 #pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-      const uint32_t oc = __shfl_up(cincl, d, WAVE);
-      const int32_t ol = __shfl_up(lincl, d, WAVE);
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const uint32_t oc = __shfl_up(cincl, d, WAVE);
+    const int32_t ol = __shfl_up(lincl, d, WAVE);
+    if (lane >= d) {
+      cincl += oc;
+      lincl = max(lincl, ol);
+    }
+  }
+  if (lane == WAVE - 1) {
+    s_wcnt[w] = cincl;
+    s_wlast[w] = lincl;
+  }
+  __syncthreads();
+  if (w == 0) {
+    uint32_t wc = lane < LEX_CT_WAVES ? s_wcnt[lane] : 0u;
+    int32_t wl = lane < LEX_CT_WAVES ? s_wlast[lane] : -1;
+    uint32_t wci = wc;
+    int32_t wli = wl;
+#pragma unroll
+    for (int d = 1; d < LEX_CT_WAVES; d <<= 1) {
+      const uint32_t oc = __shfl_up(wci, d, WAVE);
+      const int32_t ol = __shfl_up(wli, d, WAVE);
       if (lane >= d) {
-        cincl += oc;
-        lincl = max(lincl, ol);
+        wci += oc;
+        wli = max(wli, ol);
       }
     }
-    const uint32_t cexcl = cincl - cnt;
-    int32_t lexcl = __shfl_up(lincl, 1, WAVE);
-    if (lane == 0) lexcl = -1;
-    const uint32_t ctot = __shfl(cincl, WAVE - 1, WAVE);
-    const int32_t ltot = __shfl(lincl, WAVE - 1, WAVE);
-    if (lane == 0) st_relaxed64(p.st_count + tile, pack_count(ctot, ltot, tile == 0 ? CT_INC : CT_AGG));
+    const uint32_t ctot = __shfl(wci, LEX_CT_WAVES - 1, WAVE);
+    const int32_t ltot = __shfl(wli, LEX_CT_WAVES - 1, WAVE);
+    if (lane == 0) st_relaxed64(p.st_count + ct, pack_count(ctot, ltot, ct == 0 ? CT_INC : CT_AGG));
     uint32_t cpre = 0;
     int32_t lpre = -1;
-    if (tile > 0) {
-      lookback_count(p, tile, lane, cpre, lpre);
-      if (lane == 0) st_relaxed64(p.st_count + tile, pack_count(cpre + ctot, max(lpre, ltot), CT_INC));
+    if (ct > 0) {
+      lookback_count(p, ct, lane, cpre, lpre);
+      if (lane == 0) st_relaxed64(p.st_count + ct, pack_count(cpre + ctot, max(lpre, ltot), CT_INC));
     }
+    // exclusive per wave: prefix of the tile + waves before
+    int32_t wlex = __shfl_up(wli, 1, WAVE);
+    uint32_t wcex = wci - wc;
+    if (lane == 0) wlex = -1;
+    if (lane < LEX_CT_WAVES) {
+      s_wcnt[lane] = cpre + wcex;
+      s_wlast[lane] = max(lpre, wlex);
+    }
+  }
+  __syncthreads();
 
-    // ---- pass 3: write tokens
-    uint32_t out = cpre + cexcl;
-    int32_t cur_start = max(max(lpre, lexcl), 0);
-    if (valid > 0) {
-      uint32_t ent = s_dfa[s_lane * C + LEX_CLS(0)];
-      uint32_t s = ent & 0xFFu;
-      uint32_t lab = ent >> 9;
-      if (((ent >> 8) & 1u) && lab != ign) cur_start = (int32_t)base;
+  // ---- pass 2: write tokens
+  uint32_t out = s_wcnt[w] + (cincl - cnt);
+  int32_t lexcl = __shfl_up(lincl, 1, WAVE);
+  if (lane == 0) lexcl = -1;
+  int32_t cur_start = max(max(s_wlast[w], lexcl), 0);
+  if (valid > 0) {
+    uint32_t ent = s_dfa[s_lane * C + LEX_CLS(0)];
+    uint32_t s = ent & 0xFFu;
+    uint32_t lab = ent >> 9;
+    if (((ent >> 8) & 1u) && lab != ign) cur_start = (int32_t)base;
 #pragma unroll
-      for (int j = 0; j < LEX_IPT; ++j) {
-        if (j < valid) {
-          const bool inner = j + 1 < valid;
-          const bool has_next = inner || (base + j + 1 < n);
-          const uint32_t cn = inner ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
-          const uint32_t ent2 = has_next ? s_dfa[s * C + cn] : 0u;
-          const bool prod2 = (ent2 >> 8) & 1u;
-          const uint32_t lab2 = ent2 >> 9;
-          if (has_next && prod2 && lab != ign) {
-            p.kinds[out] = lab;
-            p.starts[out] = (uint32_t)cur_start;
-            p.ends[out] = (uint32_t)(base + j + 1);
-            ++out;
-          }
-          if (inner) {
-            if (prod2 && lab2 != ign) cur_start = (int32_t)(base + j + 1);
-            s = ent2 & 0xFFu;
-            lab = lab2;
-          }
-        }
-      }
-      // ---- the lane holding the last byte appends the final token and decides acceptance
-      if (base + (uint64_t)valid == n) {
-        uint32_t total = out;
-        if (lab_end != ign) {
-          p.kinds[out] = lab_end;
+    for (int j = 0; j < LEX_IPT; ++j) {
+      if (j < valid) {
+        const bool inner = j + 1 < valid;
+        const bool has_next = inner || (base + j + 1 < n);
+        const uint32_t cn = inner ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
+        const uint32_t ent2 = has_next ? s_dfa[s * C + cn] : 0u;
+        const bool prod2 = (ent2 >> 8) & 1u;
+        const uint32_t lab2 = ent2 >> 9;
+        if (has_next && prod2 && lab != ign) {
+          p.kinds[out] = lab;
           p.starts[out] = (uint32_t)cur_start;
-          p.ends[out] = (uint32_t)n;
-          total = out + 1;
+          p.ends[out] = (uint32_t)(base + j + 1);
+          ++out;
         }
-        p.result[1] = total;
-        p.result[0] = p.state_accept[s_end] ? 1u : 0u;
+        if (inner) {
+          if (prod2 && lab2 != ign) cur_start = (int32_t)(base + j + 1);
+          s = ent2 & 0xFFu;
+          lab = lab2;
+        }
       }
+    }
+    // the lane holding the last byte appends the final token and decides acceptance
+    if (base + (uint64_t)valid == n) {
+      uint32_t total = out;
+      if (lab_end != ign) {
+        p.kinds[out] = lab_end;
+        p.starts[out] = (uint32_t)cur_start;
+        p.ends[out] = (uint32_t)n;
+        total = out + 1;
+      }
+      p.result[1] = total;
+      p.result[0] = p.state_accept[s_end] ? 1u : 0u;
     }
   }
 }
@@ -647,6 +771,9 @@ inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 16) {
 
 static std::string g_create_error;
 
+enum Stage { S_LEX_REDUCE = 0, S_LEX_SCAN, S_LEX_EMIT, S_PARSE, NSTAGE };
+static const char *kStageNames[NSTAGE] = {"lex_reduce", "lex_scan", "lex_emit", "parse"};
+
 struct alpacc_ctx {
   int device = 0;
   int cus = 0;
@@ -670,24 +797,32 @@ struct alpacc_ctx {
   // scratch (grow-only)
   std::vector<std::pair<void *, size_t>> scratch;
   uint32_t *h_small = nullptr;  // pinned host words for results
-  // profiling: events around the lexer kernel and around the parser stage
+  // profiling: HIP events on the call's stream around each stage
   bool prof = false;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  bool ev_lex = false, ev_parse = false;
-  double prof_lex_ms = 0.0, prof_parse_ms = 0.0;
-  uint64_t prof_lex_calls = 0, prof_parse_calls = 0;
+  hipEvent_t ev[NSTAGE][2] = {};
+  bool rec[NSTAGE] = {};
+  double prof_ms[NSTAGE] = {};
+  uint64_t prof_calls[NSTAGE] = {};
 
+  void prof_begin(int s, hipStream_t st) {
+    if (prof) (void)hipEventRecord(ev[s][0], st);
+  }
+  void prof_end(int s, hipStream_t st) {
+    if (prof) {
+      (void)hipEventRecord(ev[s][1], st);
+      rec[s] = true;
+    }
+  }
+  void prof_drop(int s) { rec[s] = false; }
   void prof_collect() {
-    float ms = 0.f;
-    if (ev_lex && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) {
-      prof_lex_ms += ms;
-      prof_lex_calls++;
+    for (int s = 0; s < NSTAGE; ++s) {
+      float ms = 0.f;
+      if (rec[s] && hipEventElapsedTime(&ms, ev[s][0], ev[s][1]) == hipSuccess) {
+        prof_ms[s] += ms;
+        prof_calls[s]++;
+      }
+      rec[s] = false;
     }
-    if (ev_parse && hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) {
-      prof_parse_ms += ms;
-      prof_parse_calls++;
-    }
-    ev_lex = ev_parse = false;
   }
 
   void *buf(int id, size_t bytes) {
@@ -709,7 +844,11 @@ namespace {
 
 enum BufId {
   B_SMALL = 0,
-  B_ST_STATE,
+  B_LANE_EXCL,
+  B_TILE_AGG,
+  B_TILE_EXCL,
+  B_BLOCK_AGG,
+  B_TILE_STATE,
   B_ST_COUNT,
   B_TOK_KIND,
   B_TOK_START,
@@ -831,16 +970,21 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
     return ALPACC_ERR_ARG;
   }
   const uint64_t tiles = (n + LEX_TILE - 1) / LEX_TILE;
+  const uint64_t ctiles = (tiles + LEX_CT_WAVES - 1) / LEX_CT_WAVES;
+  const uint64_t sblocks = (tiles + LEX_SCAN_BLOCK - 1) / LEX_SCAN_BLOCK;
   uint32_t *small = (uint32_t *)ctx->buf(B_SMALL, 64);
-  uint32_t *st_state = (uint32_t *)ctx->buf(B_ST_STATE, tiles * sizeof(uint32_t));
-  uint64_t *st_count = (uint64_t *)ctx->buf(B_ST_COUNT, tiles * sizeof(uint64_t));
-  if (!small || !st_state || !st_count) {
+  uint16_t *lane_excl = (uint16_t *)ctx->buf(B_LANE_EXCL, tiles * WAVE * sizeof(uint16_t));
+  uint16_t *tile_agg = (uint16_t *)ctx->buf(B_TILE_AGG, tiles * sizeof(uint16_t));
+  uint16_t *tile_excl = (uint16_t *)ctx->buf(B_TILE_EXCL, tiles * sizeof(uint16_t));
+  uint16_t *block_agg = (uint16_t *)ctx->buf(B_BLOCK_AGG, sblocks * sizeof(uint16_t));
+  uint8_t *tile_state = (uint8_t *)ctx->buf(B_TILE_STATE, tiles);
+  uint64_t *st_count = (uint64_t *)ctx->buf(B_ST_COUNT, ctiles * sizeof(uint64_t));
+  if (!small || !lane_excl || !tile_agg || !tile_excl || !block_agg || !tile_state || !st_count) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
   clear_u32<<<1, 64, 0, st>>>(small, 16);
-  clear_u32<<<grid_for(tiles, 256), 256, 0, st>>>(st_state, tiles);
-  clear_u32<<<grid_for(2 * tiles, 256), 256, 0, st>>>((uint32_t *)st_count, 2 * tiles);
+  clear_u32<<<grid_for(2 * ctiles, 256), 256, 0, st>>>((uint32_t *)st_count, 2 * ctiles);
   LexArgs a;
   a.in = d_in;
   a.n = n;
@@ -859,24 +1003,34 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   a.init = ctx->init;
   a.ignore = ctx->ignore;
   a.num_tiles = (uint32_t)tiles;
+  a.num_ctiles = (uint32_t)ctiles;
   a.aligned = ((uintptr_t)d_in & 15) == 0 ? 1u : 0u;
+  a.lane_excl = lane_excl;
+  a.tile_agg = tile_agg;
+  a.tile_excl = tile_excl;
+  a.block_agg = block_agg;
+  a.tile_state = tile_state;
   a.tile_counter = small;
-  a.st_state = st_state;
   a.st_count = st_count;
   a.result = small + 4;
-  const uint64_t waves_per_block = LEX_BLOCK / WAVE;
-  uint64_t blocks = (tiles + waves_per_block - 1) / waves_per_block;
+  const uint64_t wpb = LEX_BLOCK / WAVE;
+  uint64_t blocks = (tiles + wpb - 1) / wpb;
   blocks = std::min<uint64_t>(blocks, (uint64_t)std::max(1, ctx->cus) * 2);
-  if (ctx->prof) CK(hipEventRecord(ctx->ev[0], st));
+  ctx->prof_begin(S_LEX_REDUCE, st);
   if (ctx->comp_in_lds)
-    lex_kernel<true><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
+    lex_reduce<true><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
   else
-    lex_kernel<false><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
-  if (ctx->prof) {
-    CK(hipEventRecord(ctx->ev[1], st));
-    ctx->ev_lex = true;
-  }
-  ctx->launches += 4;
+    lex_reduce<false><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
+  ctx->prof_end(S_LEX_REDUCE, st);
+  ctx->prof_begin(S_LEX_SCAN, st);
+  lex_scan_tiles<<<(unsigned)sblocks, LEX_SCAN_BLOCK, 0, st>>>(a);
+  lex_scan_blocks<<<1, LEX_SCAN_BLOCK, 0, st>>>(a, (uint32_t)sblocks);
+  lex_tile_states<<<grid_for(tiles, 256), 256, 0, st>>>(a);
+  ctx->prof_end(S_LEX_SCAN, st);
+  ctx->prof_begin(S_LEX_EMIT, st);
+  lex_emit<<<(unsigned)ctiles, LEX_BLOCK, 0, st>>>(a);
+  ctx->prof_end(S_LEX_EMIT, st);
+  ctx->launches += 7;
   CK(hipGetLastError());
   CK(hipMemcpyAsync(ctx->h_small, small + 4, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   CK(hipStreamSynchronize(st));
@@ -921,10 +1075,7 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *sta
   a.arity = ctx->d_arity;
   a.p2t = ctx->d_p2t;
   a.flags = flags;
-  if (ctx->prof) {
-    CK(hipEventRecord(ctx->ev[2], st));
-    ctx->ev_parse = true;
-  }
+  ctx->prof_begin(S_PARSE, st);
   clear_u32<<<1, 64, 0, st>>>(flags, 4);
   parse_keys<<<grid_for(P, 256), 256, 0, st>>>(a, ent, lens);
   ctx->launches += 2;
@@ -1000,7 +1151,7 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *sta
                                                 span);
   ctx->launches++;
   CK(hipGetLastError());
-  if (ctx->prof) CK(hipEventRecord(ctx->ev[3], st));
+  ctx->prof_end(S_PARSE, st);
   CK(hipStreamSynchronize(st));
   return ALPACC_ACCEPT;
 }
@@ -1054,24 +1205,29 @@ alpacc_ctx *alpacc_create(const alpacc_tables *t, int device) {
       delete ctx;
       return nullptr;
     }
-    size_t lds = ((size_t)ctx->e * ctx->C + (size_t)ctx->m1 * ctx->C) * 2 + 256;
+    if ((size_t)ctx->m1 * ctx->C > (size_t)LEX_MAX_DFA) {
+      err = "lexer DFA table too large";
+      delete ctx;
+      return nullptr;
+    }
+    size_t lds = (size_t)ctx->e * ctx->C * 2 + 256;
     lds = (lds + 15) & ~(size_t)15;
     const size_t comp_bytes = (size_t)ctx->e * ctx->e * 2;
     ctx->comp_in_lds = lds + comp_bytes <= 48 * 1024;
     if (ctx->comp_in_lds) lds += comp_bytes;
+    if (lds > 160 * 1024) {
+      err = "lexer tables do not fit in LDS";
+      delete ctx;
+      return nullptr;
+    }
     if (lds > 64 * 1024) {
       const int want = (int)lds;
-      if (hipFuncSetAttribute((const void *)lex_kernel<false>,
+      if (hipFuncSetAttribute((const void *)lex_reduce<false>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess) {
         err = "lexer tables do not fit in LDS";
         delete ctx;
         return nullptr;
       }
-    }
-    if (lds > 160 * 1024) {
-      err = "lexer tables do not fit in LDS";
-      delete ctx;
-      return nullptr;
     }
     ctx->lex_lds = lds;
   }
@@ -1109,8 +1265,9 @@ void alpacc_destroy(alpacc_ctx *ctx) {
   for (void *p : tabs)
     if (p) (void)hipFree(p);
   if (ctx->h_small) (void)hipHostFree(ctx->h_small);
-  for (auto &e : ctx->ev)
-    if (e) (void)hipEventDestroy(e);
+  for (auto &pair : ctx->ev)
+    for (auto &e : pair)
+      if (e) (void)hipEventDestroy(e);
   delete ctx;
 }
 
@@ -1120,26 +1277,34 @@ int alpacc_last_launch_count(alpacc_ctx *ctx) { return ctx ? ctx->launches : 0; 
 
 int alpacc_profile_enable(alpacc_ctx *ctx, int on) {
   if (!ctx) return ALPACC_ERR_ARG;
-  if (on && !ctx->ev[0]) {
-    for (auto &e : ctx->ev)
-      if (hipEventCreate(&e) != hipSuccess) return ALPACC_ERR_HIP;
+  if (on && !ctx->ev[0][0]) {
+    for (auto &pair : ctx->ev)
+      for (auto &e : pair)
+        if (hipEventCreate(&e) != hipSuccess) return ALPACC_ERR_HIP;
   }
   ctx->prof = on != 0;
-  ctx->prof_lex_ms = ctx->prof_parse_ms = 0.0;
-  ctx->prof_lex_calls = ctx->prof_parse_calls = 0;
+  for (int i = 0; i < NSTAGE; ++i) {
+    ctx->prof_ms[i] = 0.0;
+    ctx->prof_calls[i] = 0;
+    ctx->rec[i] = false;
+  }
   return 0;
 }
 
-int alpacc_profile_read(alpacc_ctx *ctx, double *lex_ms, uint64_t *lex_calls, double *parse_ms,
-                        uint64_t *parse_calls) {
+int alpacc_profile_read(alpacc_ctx *ctx, const char **names, double *ms, uint64_t *calls,
+                        int max_stages) {
   if (!ctx) return ALPACC_ERR_ARG;
-  if (lex_ms) *lex_ms = ctx->prof_lex_ms;
-  if (lex_calls) *lex_calls = ctx->prof_lex_calls;
-  if (parse_ms) *parse_ms = ctx->prof_parse_ms;
-  if (parse_calls) *parse_calls = ctx->prof_parse_calls;
-  ctx->prof_lex_ms = ctx->prof_parse_ms = 0.0;
-  ctx->prof_lex_calls = ctx->prof_parse_calls = 0;
-  return 0;
+  const int m = std::min(max_stages, (int)NSTAGE);
+  for (int i = 0; i < m; ++i) {
+    if (names) names[i] = kStageNames[i];
+    if (ms) ms[i] = ctx->prof_ms[i];
+    if (calls) calls[i] = ctx->prof_calls[i];
+  }
+  for (int i = 0; i < NSTAGE; ++i) {
+    ctx->prof_ms[i] = 0.0;
+    ctx->prof_calls[i] = 0;
+  }
+  return m;
 }
 
 int alpacc_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_kinds,
@@ -1154,7 +1319,6 @@ int alpacc_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_kin
   ctx->err.clear();
   (void)hipSetDevice(ctx->device);
   const int rc = run_lex(ctx, d_in, n, d_kinds, d_starts, d_ends, num_tokens, (hipStream_t)stream);
-  ctx->ev_parse = false;
   if (ctx->prof) ctx->prof_collect();
   return rc;
 }
@@ -1169,8 +1333,7 @@ int alpacc_parse_tokens(alpacc_ctx *ctx, const uint32_t *d_kinds, const uint32_t
   (void)hipSetDevice(ctx->device);
   const int rc = run_parse_tokens(ctx, d_kinds, d_starts, d_ends, num_tokens, d_parent, d_node,
                                   d_span, cap, num_nodes, (hipStream_t)stream);
-  if (rc != ALPACC_ACCEPT) ctx->ev_parse = false;
-  ctx->ev_lex = false;
+  if (rc != ALPACC_ACCEPT) ctx->prof_drop(S_PARSE);
   if (ctx->prof) ctx->prof_collect();
   return rc;
 }
@@ -1199,14 +1362,13 @@ int alpacc_parse(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_p
   uint64_t T = 0;
   int rc = run_lex(ctx, d_in, n, kinds, starts, ends, &T, st);
   if (rc != ALPACC_ACCEPT) {
-    ctx->ev_parse = false;
     if (ctx->prof) ctx->prof_collect();
     return rc;
   }
   const int lex_launches = ctx->launches;
   rc = run_parse_tokens(ctx, kinds, starts, ends, T, d_parent, d_node, d_span, cap, num_nodes, st);
   ctx->launches += lex_launches;
-  if (rc != ALPACC_ACCEPT) ctx->ev_parse = false;
+  if (rc != ALPACC_ACCEPT) ctx->prof_drop(S_PARSE);
   if (ctx->prof) ctx->prof_collect();
   return rc;
 }

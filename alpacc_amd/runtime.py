@@ -58,7 +58,7 @@ EXPORTS = ("alpacc_create", "alpacc_create_error", "alpacc_destroy", "alpacc_las
            "alpacc_copy_to_device", "alpacc_copy_to_host", "alpacc_device_memset",
            "alpacc_stream_create", "alpacc_stream_destroy", "alpacc_stream_sync",
            "alpacc_event_create", "alpacc_event_destroy", "alpacc_event_record",
-           "alpacc_event_elapsed_ms")
+           "alpacc_event_elapsed_ms", "alpacc_profile_enable", "alpacc_profile_read")
 
 _lib = None
 
@@ -100,6 +100,9 @@ def load_library():
     L.alpacc_event_record.argtypes = [vp, vp]
     L.alpacc_event_elapsed_ms.argtypes = [vp, vp]
     L.alpacc_event_elapsed_ms.restype = ctypes.c_float
+    L.alpacc_profile_enable.argtypes = [vp, ctypes.c_int]
+    L.alpacc_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64), ctypes.c_int]
     _lib = L
     return L
 
@@ -251,6 +254,21 @@ class Context:
         if arr.shape[0]:
             buf.upload(arr, self.stream)
         return buf, int(arr.shape[0])
+
+    # -- profiling ---------------------------------------------------------------------------
+    def profile(self, on: bool = True):
+        """Record HIP events around every stage of later calls (on their stream)."""
+        if self.lib.alpacc_profile_enable(self.handle, 1 if on else 0) != 0:
+            raise self._err(-2)
+
+    def read_profile(self) -> dict:
+        """{stage: (total ms, calls)} accumulated since the last read."""
+        m = 8
+        names = (ctypes.c_char_p * m)()
+        ms = (ctypes.c_double * m)()
+        calls = (ctypes.c_uint64 * m)()
+        k = self.lib.alpacc_profile_read(self.handle, names, ms, calls, m)
+        return {names[i].decode(): (ms[i], calls[i]) for i in range(k)}
 
     # -- device-level calls -----------------------------------------------------------------
     def lex_device(self, d_in: int, n: int, kinds: int, starts: int, ends: int, cap: int,

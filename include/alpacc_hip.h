@@ -103,12 +103,13 @@ int alpacc_parse_tokens(alpacc_ctx *ctx, const uint32_t *d_kinds, const uint32_t
 /* Number of kernels the last call on `ctx` launched. */
 int alpacc_last_launch_count(alpacc_ctx *ctx);
 
-/* Device-time profiling: when enabled, every call records HIP events on its stream around the
- * lexer kernel and around the parser stage; alpacc_profile_read returns the accumulated
- * milliseconds and call counts since the last read (and resets them). */
+/* Device-time profiling: when enabled, every call records HIP events on its stream around each
+ * stage (lex_reduce, lex_scan, lex_emit, parse). alpacc_profile_read fills up to max_stages
+ * names, accumulated milliseconds and call counts since the last read, resets them, and returns
+ * the number of stages filled. */
 int alpacc_profile_enable(alpacc_ctx *ctx, int on);
-int alpacc_profile_read(alpacc_ctx *ctx, double *lex_ms, uint64_t *lex_calls, double *parse_ms,
-                        uint64_t *parse_calls);
+int alpacc_profile_read(alpacc_ctx *ctx, const char **names, double *ms, uint64_t *calls,
+                        int max_stages);
 
 /* Device plumbing for hosts that have no HIP binding of their own (the Python host, ctypes
  * callers, FFI callers). They share this library's HIP runtime instance, which a caller's
