@@ -590,50 +590,91 @@ struct Levels {
   int nlev;
 };
 
+__device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, WAVE));
+  return v;
+}
+
+// minimum of every 64-element chunk: one wavefront per chunk, coalesced loads
 __global__ void min_reduce64(const int32_t *in, uint64_t n, int32_t *out, uint64_t nout) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nout;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    int32_t m = INT32_MAX;
-    const uint64_t lo = i * 64, hi = min(lo + 64, n);
-    for (uint64_t j = lo; j < hi; ++j) m = min(m, in[j]);
-    out[i] = m;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
+  for (uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nout;
+       c += wstride) {
+    const uint64_t i = c * WAVE + lane;
+    const int32_t m = wave_min_i32(i < n ? in[i] : INT32_MAX);
+    if (lane == 0) out[c] = m;
   }
 }
 
-__device__ int64_t prev_le(const Levels &lv, uint64_t i, int32_t v) {
-  int lev = 0;
-  uint64_t pos = i;
+// largest j in [lo, pos) with A[j] <= v, or -1; A is readable in whole 16-byte quads
+__device__ __forceinline__ int64_t search_back(const int32_t *A, uint64_t lo, uint64_t pos,
+                                               int32_t v) {
+  if (pos <= lo) return -1;
+  const int4 *A4 = reinterpret_cast<const int4 *>(A);
+  int64_t q = (int64_t)((pos - 1) >> 2);
+  const int64_t qlo = (int64_t)(lo >> 2);
+  const int32_t big = INT32_MAX;
+  while (q >= qlo) {
+    int4 x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x[u] = (q - u >= qlo) ? A4[q - u] : make_int4(big, big, big, big);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t b = (q - u) * 4;
+      if (b + 3 < (int64_t)pos && b + 3 >= (int64_t)lo && x[u].w <= v) return b + 3;
+      if (b + 2 < (int64_t)pos && b + 2 >= (int64_t)lo && x[u].z <= v) return b + 2;
+      if (b + 1 < (int64_t)pos && b + 1 >= (int64_t)lo && x[u].y <= v) return b + 1;
+      if (b < (int64_t)pos && b >= (int64_t)lo && x[u].x <= v) return b;
+    }
+    q -= 4;
+  }
+  return -1;
+}
+
+// "previous smaller-or-equal" beyond the element's own 64-chunk (find_previous (<=),
+// futhark/parser.fut:92-108, on a 64-ary min hierarchy): climb from chunk `chunk0`, then descend.
+__device__ int64_t prev_le_far(const Levels &lv, uint64_t chunk0, int32_t v) {
+  if (lv.nlev <= 1) return -1;
+  int lev = 1;
+  uint64_t pos = chunk0;
   int64_t found = -1;
   while (true) {
-    const int32_t *A = lv.L[lev];
     const uint64_t lo = (lev == lv.nlev - 1) ? 0 : ((pos >> 6) << 6);
-    for (uint64_t j = pos; j > lo; --j) {
-      if (A[j - 1] <= v) {
-        found = (int64_t)(j - 1);
-        break;
-      }
-    }
+    found = search_back(lv.L[lev], lo, pos, v);
     if (found >= 0) break;
     if (lev == lv.nlev - 1) return -1;
     pos >>= 6;
     ++lev;
   }
   while (lev > 0) {
-    const int32_t *A = lv.L[lev - 1];
     const uint64_t lo = (uint64_t)found * 64;
     const uint64_t hi = min(lo + 64, lv.len[lev - 1]);
-    int64_t f = -1;
-    for (uint64_t j = hi; j > lo; --j) {
-      if (A[j - 1] <= v) {
-        f = (int64_t)(j - 1);
-        break;
-      }
-    }
-    found = f;
+    found = search_back(lv.L[lev - 1], lo, hi, v);
     --lev;
     if (found < 0) return -1;
   }
   return found;
+}
+
+// Wave-cooperative previous smaller-or-equal for the 64 consecutive elements of chunk `chunk`
+// (one per lane, value v; lanes past the end hold INT32_MAX). Inside the chunk a ballot per lane
+// value finds every candidate at once; lanes with `need` and no candidate search further back.
+// All 64 lanes must call this together.
+__device__ __forceinline__ int64_t prev_le_wave(const Levels &lv, uint64_t chunk, int lane,
+                                                int32_t v, bool need) {
+  uint64_t myset = 0;
+#pragma unroll
+  for (int t = 0; t < WAVE; ++t) {
+    const int32_t vt = __builtin_amdgcn_readlane(v, t);
+    const uint64_t m = __ballot(v <= vt);
+    if (lane == t) myset = m;
+  }
+  myset &= lane == 0 ? 0ull : ((1ull << lane) - 1ull);
+  if (myset) return (int64_t)(chunk * WAVE) + 63 - __clzll((long long)myset);
+  if (!need) return -1;
+  return prev_le_far(lv, chunk, v);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -699,25 +740,43 @@ __global__ void bracket_vals(const uint16_t *br, uint64_t B, I2 *v) {
     v[i] = I2{(br[i] & 0x8000u) ? 1 : -1, 0};
 }
 
-// depths (parser.fut:153-166): reject on a negative prefix or a non-zero total
+// depths (parser.fut:153-166): reject on a negative prefix or a non-zero total; also the
+// 64-chunk minima of the depths (level 1 of the min hierarchy). One wavefront per chunk.
 __global__ void bracket_depth(const uint16_t *br, const I2 *incl, uint64_t B, int32_t *depth,
-                              uint32_t *flags) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < B;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const int32_t s = incl[i].x;
-    if (s < 0 || (i == B - 1 && s != 0)) atomicOr(flags, 1u);
-    depth[i] = s - ((br[i] & 0x8000u) ? 1 : 0);
+                              int32_t *l1, uint32_t *flags) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const uint64_t nch = (B + WAVE - 1) / WAVE;
+  const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
+  for (uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch;
+       c += wstride) {
+    const uint64_t i = c * WAVE + lane;
+    int32_t d = INT32_MAX;
+    if (i < B) {
+      const int32_t s = incl[i].x;
+      if (s < 0 || (i == B - 1 && s != 0)) atomicOr(flags, 1u);
+      d = s - ((br[i] & 0x8000u) ? 1 : 0);
+      depth[i] = d;
+    }
+    const int32_t m = wave_min_i32(d);
+    if (lane == 0) l1[c] = m;
   }
 }
 
-// brackets_matches (parser.fut:174-184)
+// brackets_matches (parser.fut:174-184): every right bracket equals (without the left bit) the
+// bracket at its previous smaller-or-equal depth. One wavefront per 64 brackets.
 __global__ void bracket_match(const uint16_t *br, uint64_t B, Levels lv, uint32_t *flags) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < B;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint16_t b = br[i];
-    if (b & 0x8000u) continue;
-    const int64_t j = prev_le(lv, i, lv.L[0][i]);
-    if (j < 0 || (br[j] & 0x7FFFu) != (b & 0x7FFFu)) atomicOr(flags, 1u);
+  const int lane = threadIdx.x & (WAVE - 1);
+  const uint64_t nch = (B + WAVE - 1) / WAVE;
+  const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
+  for (uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch;
+       c += wstride) {
+    const uint64_t i = c * WAVE + lane;
+    const bool valid = i < B;
+    const uint16_t b = valid ? br[i] : (uint16_t)0x8000u;
+    const int32_t v = valid ? lv.L[0][i] : INT32_MAX;
+    const bool right = valid && !(b & 0x8000u);
+    const int64_t j = prev_le_wave(lv, c, lane, v, right);
+    if (right && (j < 0 || (br[j] & 0x7FFFu) != (b & 0x7FFFu))) atomicOr(flags, 1u);
   }
 }
 
@@ -729,21 +788,40 @@ __global__ void prod_vals(ParseArgs a, const uint16_t *prod, uint64_t N, I2 *v) 
   }
 }
 
-__global__ void split_x(const I2 *v, uint64_t N, int32_t *x) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    x[i] = v[i].x;
+// s = exscan(arity - 1) as a plain array plus its 64-chunk minima. One wavefront per chunk.
+__global__ void split_x(const I2 *v, uint64_t N, int32_t *x, int32_t *l1) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const uint64_t nch = (N + WAVE - 1) / WAVE;
+  const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
+  for (uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch;
+       c += wstride) {
+    const uint64_t i = c * WAVE + lane;
+    int32_t s = INT32_MAX;
+    if (i < N) {
+      s = v[i].x;
+      x[i] = s;
+    }
+    const int32_t m = wave_min_i32(s);
+    if (lane == 0) l1[c] = m;
+  }
 }
 
-// parents (parser.fut:286-296) and terminal nodes (parser.fut:305-332)
+// parents (parser.fut:286-296) and terminal nodes (parser.fut:305-332). One wavefront per chunk.
 __global__ void parse_nodes(ParseArgs a, const uint16_t *prod, const I2 *ex, uint64_t N, Levels lv,
                             const uint32_t *starts, const uint32_t *ends, uint32_t *parent,
                             uint32_t *node, uint32_t *span) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
-       i += (uint64_t)gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const uint64_t nch = (N + WAVE - 1) / WAVE;
+  const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
+  for (uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch;
+       c += wstride) {
+    const uint64_t i = c * WAVE + lane;
+    const bool valid = i < N;
+    const int32_t v = valid ? lv.L[0][i] : INT32_MAX;
+    const int64_t j = prev_le_wave(lv, c, lane, v, valid && i > 0);
+    if (!valid) continue;
+    parent[i] = i == 0 ? 0u : (uint32_t)j;
     const uint16_t pr = prod[i];
-    const int64_t par = i == 0 ? 0 : prev_le(lv, i, ex[i].x);
-    parent[i] = (uint32_t)par;
     const int32_t t = a.p2t[pr];
     if (t >= 0) {
       const uint32_t r = (uint32_t)ex[i].y;
@@ -866,6 +944,8 @@ enum BufId {
   B_PVAL,
   B_PEX,
   B_SX,
+  B_DL1,
+  B_SL1,
 };
 
 template <typename T>
@@ -918,22 +998,29 @@ int scan_i2(alpacc_ctx *ctx, const I2 *in, I2 *out, uint64_t n, bool inclusive, 
   return 0;
 }
 
-int build_levels(alpacc_ctx *ctx, int32_t *base, uint64_t n, Levels &lv, hipStream_t st) {
-  uint64_t total = 0, len = n;
-  while (len > 64) {
-    len = (len + 63) / 64;
-    total += len;
+// Level arrays of the min hierarchy over `base` (n values): level 1 must already be in
+// `l1`; levels 2.. are built here. Every level is padded so that 16-byte quads can be read.
+int build_levels(alpacc_ctx *ctx, int32_t *base, int32_t *l1, uint64_t n, Levels &lv,
+                 hipStream_t st) {
+  lv.nlev = 1;
+  lv.L[0] = base;
+  lv.len[0] = n;
+  if (n <= 64) return 0;
+  uint64_t len = (n + 63) / 64;
+  lv.L[1] = l1;
+  lv.len[1] = len;
+  lv.nlev = 2;
+  uint64_t total = 0, l = len;
+  while (l > 64) {
+    l = (l + 63) / 64;
+    total += ((l + 3) & ~3ull) + 16;
   }
-  int32_t *area = (int32_t *)ctx->buf(B_LEVELS, (total + 1) * sizeof(int32_t));
+  int32_t *area = (int32_t *)ctx->buf(B_LEVELS, (total + 16) * sizeof(int32_t));
   if (!area) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
-  lv.nlev = 1;
-  lv.L[0] = base;
-  lv.len[0] = n;
-  int32_t *cur = base;
-  len = n;
+  int32_t *cur = l1;
   uint64_t off = 0;
   while (len > 64) {
     if (lv.nlev >= MAX_LEVELS) {
@@ -942,12 +1029,12 @@ int build_levels(alpacc_ctx *ctx, int32_t *base, uint64_t n, Levels &lv, hipStre
     }
     const uint64_t nl = (len + 63) / 64;
     int32_t *next = area + off;
-    min_reduce64<<<grid_for(nl, 256), 256, 0, st>>>(cur, len, next, nl);
+    min_reduce64<<<grid_for(nl * WAVE, 256), 256, 0, st>>>(cur, len, next, nl);
     ctx->launches++;
     lv.L[lv.nlev] = next;
     lv.len[lv.nlev] = nl;
     lv.nlev++;
-    off += nl;
+    off += ((nl + 3) & ~3ull) + 16;
     cur = next;
     len = nl;
   }
@@ -1099,8 +1186,9 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *sta
   if (B > 0) {
     I2 *bval = (I2 *)ctx->buf(B_BVAL, B * sizeof(I2));
     I2 *bincl = (I2 *)ctx->buf(B_BINCL, B * sizeof(I2));
-    int32_t *depth = (int32_t *)ctx->buf(B_DEPTH, B * sizeof(int32_t));
-    if (!bval || !bincl || !depth) {
+    int32_t *depth = (int32_t *)ctx->buf(B_DEPTH, (B + 16) * sizeof(int32_t));
+    int32_t *dl1 = (int32_t *)ctx->buf(B_DL1, ((B + 63) / 64 + 16) * sizeof(int32_t));
+    if (!bval || !bincl || !depth || !dl1) {
       ctx->err = "scratch allocation failed";
       return ALPACC_ERR_HIP;
     }
@@ -1108,10 +1196,10 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *sta
     ctx->launches++;
     rc = scan_i2(ctx, bval, bincl, B, true, nullptr, st);
     if (rc) return rc;
-    bracket_depth<<<grid_for(B, 256), 256, 0, st>>>(br, bincl, B, depth, flags);
+    bracket_depth<<<grid_for(B, 256), 256, 0, st>>>(br, bincl, B, depth, dl1, flags);
     ctx->launches++;
     Levels lv;
-    rc = build_levels(ctx, depth, B, lv, st);
+    rc = build_levels(ctx, depth, dl1, B, lv, st);
     if (rc) return rc;
     bracket_match<<<grid_for(B, 256), 256, 0, st>>>(br, B, lv, flags);
     ctx->launches++;
@@ -1128,8 +1216,9 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *sta
   if (N == 0) return ALPACC_ACCEPT;
   I2 *pval = (I2 *)ctx->buf(B_PVAL, N * sizeof(I2));
   I2 *pex = (I2 *)ctx->buf(B_PEX, N * sizeof(I2));
-  int32_t *sx = (int32_t *)ctx->buf(B_SX, N * sizeof(int32_t));
-  if (!pval || !pex || !sx) {
+  int32_t *sx = (int32_t *)ctx->buf(B_SX, (N + 16) * sizeof(int32_t));
+  int32_t *sl1 = (int32_t *)ctx->buf(B_SL1, ((N + 63) / 64 + 16) * sizeof(int32_t));
+  if (!pval || !pex || !sx || !sl1) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
@@ -1142,10 +1231,10 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *sta
     ctx->err = "terminal productions do not match the token count";
     return ALPACC_ERR_ARG;
   }
-  split_x<<<grid_for(N, 256), 256, 0, st>>>(pex, N, sx);
+  split_x<<<grid_for(N, 256), 256, 0, st>>>(pex, N, sx, sl1);
   ctx->launches++;
   Levels lv;
-  rc = build_levels(ctx, sx, N, lv, st);
+  rc = build_levels(ctx, sx, sl1, N, lv, st);
   if (rc) return rc;
   parse_nodes<<<grid_for(N, 256), 256, 0, st>>>(a, prod, pex, N, lv, starts, ends, parent, node,
                                                 span);
