@@ -39,7 +39,8 @@ constexpr uint64_t CT_INC = 2ull << 62;
 constexpr uint32_t SPIN_LIMIT = 1u << 22;        // bounded waits: a stuck look-back ends the kernel
 constexpr int LEX_CT_WAVES = LEX_BLOCK / WAVE;   // wave tiles per count tile (one block)
 constexpr int LEX_SCAN_BLOCK = 1024;             // wave tiles per K2a scan block
-constexpr int LEX_MAX_DFA = 8192;                // max (m+1)*C entries of the DFA table in LDS
+constexpr int LEX_MAX_DFA = 4096;                // max (m+1)*C entries of the DFA table in LDS
+constexpr int LEX_STAGE = 256;                   // tokens staged per wavefront and round
 
 // ----------------------------------------------------------------------------------------------
 // small device helpers
@@ -65,6 +66,11 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
 __device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, WAVE));
+  return v;
+}
+__device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, WAVE));
   return v;
 }
 
@@ -322,8 +328,7 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   __shared__ uint32_t s_wcnt[LEX_CT_WAVES];
   __shared__ int32_t s_wlast[LEX_CT_WAVES];
   __shared__ uint32_t s_ctile;
-  __shared__ uint32_t s_cpre;
-  __shared__ int32_t s_lpre;
+  __shared__ uint32_t s_stage[LEX_CT_WAVES][3][LEX_STAGE];
   const uint32_t C = p.C;
   for (uint32_t i = threadIdx.x; i < p.m1 * C; i += blockDim.x) s_dfa[i] = p.dfa_step[i];
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_cls[i] = p.byte_class[i];
@@ -352,28 +357,29 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
     c_next = nb < n ? s_cls[p.in[nb]] : 0u;
   }
 
-  // ---- pass 1: count token ends and find the last token start
-  uint32_t cnt = 0;
-  int32_t last = -1;
+  // ---- one walk: token-end mask, token-start mask and the label after every byte
+  uint32_t endm = 0, startm = 0;
+  uint32_t labs[LEX_IPT / 4] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t s_end = s_lane;
   uint32_t lab_end = 0;
   if (valid > 0) {
     uint32_t ent = s_dfa[s_lane * C + LEX_CLS(0)];
     uint32_t s = ent & 0xFFu;
     uint32_t lab = ent >> 9;
-    if (((ent >> 8) & 1u) && lab != ign) last = (int32_t)base;
+    if (((ent >> 8) & 1u) && lab != ign) startm |= 1u;
 #pragma unroll
     for (int j = 0; j < LEX_IPT; ++j) {
       if (j < valid) {
+        labs[j >> 2] |= (lab & 0xFFu) << (8 * (j & 3));
         const bool inner = j + 1 < valid;
         const bool has_next = inner || (base + j + 1 < n);
         const uint32_t cn = inner ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
         const uint32_t ent2 = has_next ? s_dfa[s * C + cn] : 0u;
         const bool prod2 = (ent2 >> 8) & 1u;
         const uint32_t lab2 = ent2 >> 9;
-        cnt += (has_next && prod2 && lab != ign) ? 1u : 0u;
+        if (has_next && prod2 && lab != ign) endm |= 1u << j;
         if (inner) {
-          if (prod2 && lab2 != ign) last = (int32_t)(base + j + 1);
+          if (prod2 && lab2 != ign) startm |= 1u << (j + 1);
           s = ent2 & 0xFFu;
           lab = lab2;
         }
@@ -382,6 +388,8 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
     s_end = s;
     lab_end = lab;
   }
+  const uint32_t cnt = __popc(endm);
+  const int32_t last = startm ? (int32_t)(base + 31 - __clz(startm)) : -1;
 
   // ---- scans: wave, count tile, across count tiles
   uint32_t cincl = cnt;
@@ -423,9 +431,8 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
       lookback_count(p, ct, lane, cpre, lpre);
       if (lane == 0) st_relaxed64(p.st_count + ct, pack_count(cpre + ctot, max(lpre, ltot), CT_INC));
     }
-    // exclusive per wave: prefix of the tile + waves before
     int32_t wlex = __shfl_up(wli, 1, WAVE);
-    uint32_t wcex = wci - wc;
+    const uint32_t wcex = wci - wc;
     if (lane == 0) wlex = -1;
     if (lane < LEX_CT_WAVES) {
       s_wcnt[lane] = cpre + wcex;
@@ -434,50 +441,75 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   }
   __syncthreads();
 
-  // ---- pass 2: write tokens
-  uint32_t out = s_wcnt[w] + (cincl - cnt);
+  // ---- write tokens: stage in LDS by wave-local index, copy out coalesced, in rounds
+  const uint32_t wout = s_wcnt[w];                  // global index of the wave's first token
+  const uint32_t wtotal = __shfl(cincl, WAVE - 1, WAVE);
+  const uint32_t lofs = cincl - cnt;                // wave-local index of this lane's first token
   int32_t lexcl = __shfl_up(lincl, 1, WAVE);
   if (lane == 0) lexcl = -1;
-  int32_t cur_start = max(max(s_wlast[w], lexcl), 0);
-  if (valid > 0) {
-    uint32_t ent = s_dfa[s_lane * C + LEX_CLS(0)];
-    uint32_t s = ent & 0xFFu;
-    uint32_t lab = ent >> 9;
-    if (((ent >> 8) & 1u) && lab != ign) cur_start = (int32_t)base;
-#pragma unroll
-    for (int j = 0; j < LEX_IPT; ++j) {
-      if (j < valid) {
-        const bool inner = j + 1 < valid;
-        const bool has_next = inner || (base + j + 1 < n);
-        const uint32_t cn = inner ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
-        const uint32_t ent2 = has_next ? s_dfa[s * C + cn] : 0u;
-        const bool prod2 = (ent2 >> 8) & 1u;
-        const uint32_t lab2 = ent2 >> 9;
-        if (has_next && prod2 && lab != ign) {
-          p.kinds[out] = lab;
-          p.starts[out] = (uint32_t)cur_start;
-          p.ends[out] = (uint32_t)(base + j + 1);
-          ++out;
-        }
-        if (inner) {
-          if (prod2 && lab2 != ign) cur_start = (int32_t)(base + j + 1);
-          s = ent2 & 0xFFu;
-          lab = lab2;
-        }
-      }
+  const int32_t start_in = max(max(s_wlast[w], lexcl), 0);
+  uint32_t *stk = s_stage[w][0];
+  uint32_t *sts = s_stage[w][1];
+  uint32_t *ste = s_stage[w][2];
+  for (uint32_t r0 = 0; r0 < wtotal; r0 += LEX_STAGE) {
+    // this lane's tokens with wave-local index in [r0, r0 + LEX_STAGE)
+    uint32_t em = endm;
+    uint32_t k = lofs;
+    if (k < r0) {  // skip tokens staged in earlier rounds
+      const uint32_t skip = min(r0 - k, (uint32_t)__popc(em));
+      for (uint32_t q = 0; q < skip; ++q) em &= em - 1;
+      k += skip;
     }
-    // the lane holding the last byte appends the final token and decides acceptance
-    if (base + (uint64_t)valid == n) {
-      uint32_t total = out;
-      if (lab_end != ign) {
-        p.kinds[out] = lab_end;
-        p.starts[out] = (uint32_t)cur_start;
-        p.ends[out] = (uint32_t)n;
-        total = out + 1;
+    while (em && k < r0 + LEX_STAGE) {
+      const int j = __ffs(em) - 1;
+      em &= em - 1;
+      const uint32_t sm = startm & (j == 31 ? 0xFFFFFFFFu : ((2u << j) - 1u));
+      const int32_t cur = sm ? (int32_t)(base + 31 - __clz(sm)) : start_in;
+      uint32_t lw;
+      switch (j >> 2) {
+        case 0: lw = labs[0]; break;
+        case 1: lw = labs[1]; break;
+        case 2: lw = labs[2]; break;
+        case 3: lw = labs[3]; break;
+        case 4: lw = labs[4]; break;
+        case 5: lw = labs[5]; break;
+        case 6: lw = labs[6]; break;
+        default: lw = labs[7]; break;
       }
-      p.result[1] = total;
-      p.result[0] = p.state_accept[s_end] ? 1u : 0u;
+      const uint32_t o = k - r0;
+      stk[o] = (lw >> (8 * (j & 3))) & 0xFFu;
+      sts[o] = (uint32_t)cur;
+      ste[o] = (uint32_t)(base + j + 1);
+      ++k;
     }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t m = min((uint32_t)LEX_STAGE, wtotal - r0);
+    for (uint32_t i = lane; i < m; i += WAVE) {
+      const uint64_t g = (uint64_t)wout + r0 + i;
+      p.kinds[g] = stk[i];
+      p.starts[g] = sts[i];
+      p.ends[g] = ste[i];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  // ---- the lane holding the last byte appends the final token and decides acceptance
+  if (valid > 0 && base + (uint64_t)valid == n) {
+    const uint32_t out = wout + wtotal;
+    int32_t cur = start_in;
+    if (startm) cur = (int32_t)(base + 31 - __clz(startm));
+    uint32_t total = out;
+    if (lab_end != ign) {
+      p.kinds[out] = lab_end;
+      p.starts[out] = (uint32_t)cur;
+      p.ends[out] = (uint32_t)n;
+      total = out + 1;
+    }
+    p.result[1] = total;
+    p.result[0] = p.state_accept[s_end] ? 1u : 0u;
   }
 }
 
@@ -488,11 +520,14 @@ __global__ void clear_u32(uint32_t *p, uint64_t n) {
 }
 
 // ----------------------------------------------------------------------------------------------
-// device-wide scans of int2 (two independent int32 sums), reduce-then-scan
+// device-wide exclusive/inclusive scans of int2 (two independent int32 sums), reduce-then-scan.
+// Loads are coalesced: a wavefront owns 16 rows of 64 consecutive elements. Values come from a
+// source functor and results go to a sink functor, so producers and consumers fuse into the scan.
 
 constexpr int SCAN_BLOCK = 256;
-constexpr int SCAN_ITEMS = 16;
-constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+constexpr int SCAN_ROWS = 8;
+constexpr int SCAN_WAVE_ITEMS = WAVE * SCAN_ROWS;
+constexpr int SCAN_TILE = (SCAN_BLOCK / WAVE) * SCAN_WAVE_ITEMS;
 
 struct I2 {
   int32_t x, y;
@@ -500,81 +535,174 @@ struct I2 {
 
 __device__ __forceinline__ I2 add2(I2 a, I2 b) { return I2{a.x + b.x, a.y + b.y}; }
 
-// block exclusive scan of one value per thread; returns the exclusive prefix, total in `total`
-__device__ I2 block_exclusive_scan(I2 v, I2 &total) {
+__device__ __forceinline__ I2 wave_sum2(I2 v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    v.x += __shfl_xor(v.x, d, WAVE);
+    v.y += __shfl_xor(v.y, d, WAVE);
+  }
+  return v;
+}
+
+__device__ __forceinline__ I2 wave_incl2(I2 v, int lane) {
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const int32_t ox = __shfl_up(v.x, d, WAVE);
+    const int32_t oy = __shfl_up(v.y, d, WAVE);
+    if (lane >= d) {
+      v.x += ox;
+      v.y += oy;
+    }
+  }
+  return v;
+}
+
+struct SrcArray {
+  const I2 *a;
+  __device__ I2 operator()(uint64_t i) const { return a[i]; }
+};
+
+// +1 for a left (pushed) bracket, -1 for a right (popped) one (depths, parser.fut:153-166)
+struct SrcBracket {
+  const uint16_t *br;
+  __device__ I2 operator()(uint64_t i) const { return I2{(br[i] & 0x8000u) ? 1 : -1, 0}; }
+};
+
+// (arity - 1, is terminal production) (parents, parser.fut:286-296; terminal_offsets :305-314)
+struct SrcProd {
+  const uint16_t *prod;
+  const int32_t *arity;
+  const int32_t *p2t;
+  __device__ I2 operator()(uint64_t i) const {
+    const uint16_t p = prod[i];
+    return I2{arity[p] - 1, p2t[p] >= 0 ? 1 : 0};
+  }
+};
+
+struct SinkArray {  // exclusive prefix to an array
+  I2 *out;
+  __device__ void operator()(uint64_t i, bool valid, I2 v, I2 ex, I2 in, int lane) const {
+    if (valid) out[i] = ex;
+  }
+};
+
+// bracket depth = inclusive scan - [left]; reject on a negative prefix or a non-zero total;
+// 64-element minima of the depths (level 1 of the min hierarchy).
+struct SinkDepth {
+  int32_t *depth;
+  int32_t *l1;
+  uint32_t *flags;
+  uint64_t n;
+  __device__ void operator()(uint64_t i, bool valid, I2 v, I2 ex, I2 in, int lane) const {
+    int32_t d = INT32_MAX;
+    if (valid) {
+      if (in.x < 0 || (i == n - 1 && in.x != 0)) atomicOr(flags, 1u);
+      d = in.x - (v.x > 0 ? 1 : 0);
+      depth[i] = d;
+    }
+    const int32_t m = wave_min_i32(d);
+    if (lane == 0 && i < n) l1[i / WAVE] = m;
+  }
+};
+
+// s = exscan(arity - 1), terminal rank = exscan(is terminal); minima of s.
+struct SinkProd {
+  int32_t *s;
+  int32_t *rank;
+  int32_t *l1;
+  uint64_t n;
+  __device__ void operator()(uint64_t i, bool valid, I2 v, I2 ex, I2 in, int lane) const {
+    int32_t x = INT32_MAX;
+    if (valid) {
+      x = ex.x;
+      s[i] = x;
+      rank[i] = ex.y;
+    }
+    const int32_t m = wave_min_i32(x);
+    if (lane == 0 && i < n) l1[i / WAVE] = m;
+  }
+};
+
+template <class Src>
+__global__ void __launch_bounds__(SCAN_BLOCK) scan_reduce(Src src, uint64_t n, I2 *bsum) {
+  __shared__ I2 wsum[SCAN_BLOCK / WAVE];
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  I2 s{0, 0};
+#pragma unroll 4
+  for (int r = 0; r < SCAN_TILE / SCAN_BLOCK; ++r) {
+    const uint64_t i = base + (uint64_t)r * SCAN_BLOCK + threadIdx.x;
+    if (i < n) s = add2(s, src(i));
+  }
+  s = wave_sum2(s);
+  if ((threadIdx.x & (WAVE - 1)) == 0) wsum[threadIdx.x / WAVE] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    I2 t{0, 0};
+    for (int w = 0; w < SCAN_BLOCK / WAVE; ++w) t = add2(t, wsum[w]);
+    bsum[blockIdx.x] = t;
+  }
+}
+
+// one block: exclusive scan of the block sums in place; the grand total goes to bsum[nb]
+__global__ void __launch_bounds__(SCAN_BLOCK) scan_blocks(I2 *bsum, uint64_t nb) {
   __shared__ I2 wsum[SCAN_BLOCK / WAVE];
   const int lane = threadIdx.x & (WAVE - 1);
   const int w = threadIdx.x / WAVE;
-  I2 incl = v;
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) {
-    I2 o;
-    o.x = __shfl_up(incl.x, d, WAVE);
-    o.y = __shfl_up(incl.y, d, WAVE);
-    if (lane >= d) incl = add2(incl, o);
-  }
-  if (lane == WAVE - 1) wsum[w] = incl;
-  __syncthreads();
-  I2 woff{0, 0};
-  total = I2{0, 0};
-  for (int i = 0; i < SCAN_BLOCK / WAVE; ++i) {
-    if (i < w) woff = add2(woff, wsum[i]);
-    total = add2(total, wsum[i]);
-  }
-  __syncthreads();
-  I2 ex = add2(woff, incl);
-  ex.x -= v.x;
-  ex.y -= v.y;
-  return ex;
-}
-
-__global__ void __launch_bounds__(SCAN_BLOCK) scan_reduce(const I2 *in, uint64_t n, I2 *bsum) {
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
-  I2 s{0, 0};
-  for (int i = 0; i < SCAN_ITEMS; ++i)
-    if (base + i < n) s = add2(s, in[base + i]);
-  I2 total;
-  block_exclusive_scan(s, total);
-  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
-}
-
-// one block: exclusive scan of the block sums in place; writes the grand total to bsum[nb]
-__global__ void __launch_bounds__(SCAN_BLOCK) scan_blocks(I2 *bsum, uint64_t nb) {
   I2 carry{0, 0};
   for (uint64_t off = 0; off < nb; off += SCAN_BLOCK) {
     const uint64_t i = off + threadIdx.x;
     const I2 v = i < nb ? bsum[i] : I2{0, 0};
-    I2 total;
-    const I2 ex = block_exclusive_scan(v, total);
-    if (i < nb) bsum[i] = add2(carry, ex);
-    carry = add2(carry, total);
+    const I2 incl = wave_incl2(v, lane);
+    if (lane == WAVE - 1) wsum[w] = incl;
+    __syncthreads();
+    I2 woff{0, 0}, tot{0, 0};
+    for (int k = 0; k < SCAN_BLOCK / WAVE; ++k) {
+      if (k < w) woff = add2(woff, wsum[k]);
+      tot = add2(tot, wsum[k]);
+    }
+    __syncthreads();
+    if (i < nb) {
+      I2 ex = add2(add2(carry, woff), incl);
+      ex.x -= v.x;
+      ex.y -= v.y;
+      bsum[i] = ex;
+    }
+    carry = add2(carry, tot);
   }
   if (threadIdx.x == 0) bsum[nb] = carry;
 }
 
-__global__ void __launch_bounds__(SCAN_BLOCK) scan_down(const I2 *in, uint64_t n, const I2 *bsum,
-                                                        I2 *out, int inclusive) {
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
-  I2 vals[SCAN_ITEMS];
-  I2 s{0, 0};
+template <class Src, class Sink>
+__global__ void __launch_bounds__(SCAN_BLOCK) scan_down(Src src, uint64_t n, const I2 *bsum,
+                                                        Sink sink) {
+  __shared__ I2 wsum[SCAN_BLOCK / WAVE];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = threadIdx.x / WAVE;
+  const uint64_t wbase = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)w * SCAN_WAVE_ITEMS;
+  I2 vals[SCAN_ROWS];
+  I2 tot{0, 0};
 #pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; ++i) {
-    vals[i] = base + i < n ? in[base + i] : I2{0, 0};
-    s = add2(s, vals[i]);
+  for (int r = 0; r < SCAN_ROWS; ++r) {
+    const uint64_t i = wbase + (uint64_t)r * WAVE + lane;
+    vals[r] = i < n ? src(i) : I2{0, 0};
+    tot = add2(tot, vals[r]);
   }
-  I2 total;
-  I2 run = add2(bsum[blockIdx.x], block_exclusive_scan(s, total));
+  tot = wave_sum2(tot);
+  if (lane == 0) wsum[w] = tot;
+  __syncthreads();
+  I2 carry = bsum[blockIdx.x];
+  for (int k = 0; k < w; ++k) carry = add2(carry, wsum[k]);
 #pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; ++i) {
-    if (base + i < n) {
-      if (inclusive) {
-        run = add2(run, vals[i]);
-        out[base + i] = run;
-      } else {
-        out[base + i] = run;
-        run = add2(run, vals[i]);
-      }
-    }
+  for (int r = 0; r < SCAN_ROWS; ++r) {
+    const uint64_t i = wbase + (uint64_t)r * WAVE + lane;
+    const I2 incl_row = wave_incl2(vals[r], lane);
+    const I2 in = add2(carry, incl_row);
+    I2 ex = in;
+    ex.x -= vals[r].x;
+    ex.y -= vals[r].y;
+    sink(i, i < n, vals[r], ex, in, lane);
+    carry.x += __shfl(incl_row.x, WAVE - 1, WAVE);
+    carry.y += __shfl(incl_row.y, WAVE - 1, WAVE);
   }
 }
 
@@ -589,12 +717,6 @@ struct Levels {
   uint64_t len[MAX_LEVELS];
   int nlev;
 };
-
-__device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, WAVE));
-  return v;
-}
 
 // minimum of every 64-element chunk: one wavefront per chunk, coalesced loads
 __global__ void min_reduce64(const int32_t *in, uint64_t n, int32_t *out, uint64_t nout) {
@@ -633,8 +755,7 @@ __device__ __forceinline__ int64_t search_back(const int32_t *A, uint64_t lo, ui
   return -1;
 }
 
-// "previous smaller-or-equal" beyond the element's own 64-chunk (find_previous (<=),
-// futhark/parser.fut:92-108, on a 64-ary min hierarchy): climb from chunk `chunk0`, then descend.
+// per-lane climb through levels 1.. from chunk `chunk0`, then descent (rarely needed)
 __device__ int64_t prev_le_far(const Levels &lv, uint64_t chunk0, int32_t v) {
   if (lv.nlev <= 1) return -1;
   int lev = 1;
@@ -658,23 +779,58 @@ __device__ int64_t prev_le_far(const Levels &lv, uint64_t chunk0, int32_t v) {
   return found;
 }
 
+__device__ __forceinline__ int hibit64(uint64_t m) { return 63 - __clzll((long long)m); }
+
 // Wave-cooperative previous smaller-or-equal for the 64 consecutive elements of chunk `chunk`
-// (one per lane, value v; lanes past the end hold INT32_MAX). Inside the chunk a ballot per lane
-// value finds every candidate at once; lanes with `need` and no candidate search further back.
-// All 64 lanes must call this together.
-__device__ __forceinline__ int64_t prev_le_wave(const Levels &lv, uint64_t chunk, int lane,
-                                                int32_t v, bool need) {
-  uint64_t myset = 0;
-#pragma unroll
-  for (int t = 0; t < WAVE; ++t) {
-    const int32_t vt = __builtin_amdgcn_readlane(v, t);
-    const uint64_t m = __ballot(v <= vt);
-    if (lane == t) myset = m;
+// (lane l holds element chunk*64+l with value v; lanes past the end hold INT32_MAX). Steps:
+// the previous element; a ballot per unresolved lane inside the chunk; one coalesced load of the
+// 64 preceding chunk minima and a ballot per still unresolved lane, then a ballot over the chosen
+// chunk; a per-lane climb for what is left. All 64 lanes must call this together.
+__device__ int64_t prev_le_wave(const Levels &lv, uint64_t chunk, int lane, int32_t v, bool need) {
+  const int64_t cbase = (int64_t)(chunk * WAVE);
+  int64_t res = -1;
+  bool done = !need;
+  const int32_t vp = __shfl_up(v, 1, WAVE);
+  if (!done && lane > 0 && vp <= v) {
+    res = cbase + lane - 1;
+    done = true;
   }
-  myset &= lane == 0 ? 0ull : ((1ull << lane) - 1ull);
-  if (myset) return (int64_t)(chunk * WAVE) + 63 - __clzll((long long)myset);
-  if (!need) return -1;
-  return prev_le_far(lv, chunk, v);
+  uint64_t todo = __ballot(!done);
+  uint64_t it = todo;
+  while (it) {
+    const int t = __ffsll((long long)it) - 1;
+    it &= it - 1;
+    const int32_t vt = __builtin_amdgcn_readlane(v, t);
+    const uint64_t m = __ballot(v <= vt) & ((1ull << t) - 1ull);
+    if (lane == t && m) {
+      res = cbase + hibit64(m);
+      done = true;
+    }
+  }
+  todo = __ballot(!done);
+  if (!todo) return res;
+  if (lv.nlev >= 2) {
+    const int64_t wc = (int64_t)chunk - WAVE + lane;
+    const int32_t wv = wc >= 0 ? lv.L[1][wc] : INT32_MAX;
+    it = todo;
+    while (it) {
+      const int t = __ffsll((long long)it) - 1;
+      it &= it - 1;
+      const int32_t vt = __builtin_amdgcn_readlane(v, t);
+      const uint64_t m = __ballot(wv <= vt);
+      if (m) {
+        const int64_t cb = (int64_t)chunk - WAVE + hibit64(m);
+        const int32_t x = lv.L[0][cb * WAVE + lane];
+        const uint64_t m2 = __ballot(x <= vt);
+        if (lane == t && m2) {
+          res = cb * WAVE + hibit64(m2);
+          done = true;
+        }
+      }
+    }
+  }
+  if (!done) res = prev_le_far(lv, chunk, v);
+  return res;
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -734,34 +890,6 @@ __global__ void parse_expand(ParseArgs a, const uint32_t *ent, const I2 *offs, u
   }
 }
 
-__global__ void bracket_vals(const uint16_t *br, uint64_t B, I2 *v) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < B;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    v[i] = I2{(br[i] & 0x8000u) ? 1 : -1, 0};
-}
-
-// depths (parser.fut:153-166): reject on a negative prefix or a non-zero total; also the
-// 64-chunk minima of the depths (level 1 of the min hierarchy). One wavefront per chunk.
-__global__ void bracket_depth(const uint16_t *br, const I2 *incl, uint64_t B, int32_t *depth,
-                              int32_t *l1, uint32_t *flags) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const uint64_t nch = (B + WAVE - 1) / WAVE;
-  const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
-  for (uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch;
-       c += wstride) {
-    const uint64_t i = c * WAVE + lane;
-    int32_t d = INT32_MAX;
-    if (i < B) {
-      const int32_t s = incl[i].x;
-      if (s < 0 || (i == B - 1 && s != 0)) atomicOr(flags, 1u);
-      d = s - ((br[i] & 0x8000u) ? 1 : 0);
-      depth[i] = d;
-    }
-    const int32_t m = wave_min_i32(d);
-    if (lane == 0) l1[c] = m;
-  }
-}
-
 // brackets_matches (parser.fut:174-184): every right bracket equals (without the left bit) the
 // bracket at its previous smaller-or-equal depth. One wavefront per 64 brackets.
 __global__ void bracket_match(const uint16_t *br, uint64_t B, Levels lv, uint32_t *flags) {
@@ -780,36 +908,10 @@ __global__ void bracket_match(const uint16_t *br, uint64_t B, Levels lv, uint32_
   }
 }
 
-__global__ void prod_vals(ParseArgs a, const uint16_t *prod, uint64_t N, I2 *v) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint16_t pr = prod[i];
-    v[i] = I2{a.arity[pr] - 1, a.p2t[pr] >= 0 ? 1 : 0};
-  }
-}
-
-// s = exscan(arity - 1) as a plain array plus its 64-chunk minima. One wavefront per chunk.
-__global__ void split_x(const I2 *v, uint64_t N, int32_t *x, int32_t *l1) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const uint64_t nch = (N + WAVE - 1) / WAVE;
-  const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
-  for (uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch;
-       c += wstride) {
-    const uint64_t i = c * WAVE + lane;
-    int32_t s = INT32_MAX;
-    if (i < N) {
-      s = v[i].x;
-      x[i] = s;
-    }
-    const int32_t m = wave_min_i32(s);
-    if (lane == 0) l1[c] = m;
-  }
-}
-
 // parents (parser.fut:286-296) and terminal nodes (parser.fut:305-332). One wavefront per chunk.
-__global__ void parse_nodes(ParseArgs a, const uint16_t *prod, const I2 *ex, uint64_t N, Levels lv,
-                            const uint32_t *starts, const uint32_t *ends, uint32_t *parent,
-                            uint32_t *node, uint32_t *span) {
+__global__ void parse_nodes(ParseArgs a, const uint16_t *prod, const int32_t *rank, uint64_t N,
+                            Levels lv, const uint32_t *starts, const uint32_t *ends,
+                            uint32_t *parent, uint32_t *node, uint32_t *span) {
   const int lane = threadIdx.x & (WAVE - 1);
   const uint64_t nch = (N + WAVE - 1) / WAVE;
   const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
@@ -823,16 +925,15 @@ __global__ void parse_nodes(ParseArgs a, const uint16_t *prod, const I2 *ex, uin
     parent[i] = i == 0 ? 0u : (uint32_t)j;
     const uint16_t pr = prod[i];
     const int32_t t = a.p2t[pr];
+    uint2 sp = make_uint2(0u, 0u);
     if (t >= 0) {
-      const uint32_t r = (uint32_t)ex[i].y;
+      const uint32_t r = (uint32_t)rank[i];
       node[i] = ALPACC_NODE_TERMINAL | (uint32_t)t;
-      span[2 * i] = starts[r];
-      span[2 * i + 1] = ends[r];
+      sp = make_uint2(starts[r], ends[r]);
     } else {
       node[i] = pr;
-      span[2 * i] = 0;
-      span[2 * i + 1] = 0;
     }
+    reinterpret_cast<uint2 *>(span)[i] = sp;
   }
 }
 
@@ -937,12 +1038,9 @@ enum BufId {
   B_BSUM,
   B_BR,
   B_PROD,
-  B_BVAL,
-  B_BINCL,
   B_DEPTH,
   B_LEVELS,
-  B_PVAL,
-  B_PEX,
+  B_RANK,
   B_SX,
   B_DL1,
   B_SL1,
@@ -972,8 +1070,10 @@ T *upload(const T *h, size_t count, std::string &err) {
     }                                                                   \
   } while (0)
 
-int scan_i2(alpacc_ctx *ctx, const I2 *in, I2 *out, uint64_t n, bool inclusive, I2 *total,
-            hipStream_t st) {
+// reduce-then-scan of n int2 values from `src` into `sink`; the grand total goes to *total
+// (copied to the host when total is not NULL).
+template <class Src, class Sink>
+int scan2(alpacc_ctx *ctx, Src src, Sink sink, uint64_t n, I2 *total, hipStream_t st) {
   const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   I2 *bsum = (I2 *)ctx->buf(B_BSUM, (nb + 1) * sizeof(I2));
   if (!bsum) {
@@ -981,9 +1081,9 @@ int scan_i2(alpacc_ctx *ctx, const I2 *in, I2 *out, uint64_t n, bool inclusive, 
     return ALPACC_ERR_HIP;
   }
   if (n > 0) {
-    scan_reduce<<<(unsigned)nb, SCAN_BLOCK, 0, st>>>(in, n, bsum);
+    scan_reduce<Src><<<(unsigned)nb, SCAN_BLOCK, 0, st>>>(src, n, bsum);
     scan_blocks<<<1, SCAN_BLOCK, 0, st>>>(bsum, nb);
-    scan_down<<<(unsigned)nb, SCAN_BLOCK, 0, st>>>(in, n, bsum, out, inclusive ? 1 : 0);
+    scan_down<Src, Sink><<<(unsigned)nb, SCAN_BLOCK, 0, st>>>(src, n, bsum, sink);
     ctx->launches += 3;
     CK(hipGetLastError());
   }
@@ -1168,7 +1268,7 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *sta
   ctx->launches += 2;
   CK(hipGetLastError());
   I2 tot;
-  int rc = scan_i2(ctx, lens, offs, P, false, &tot, st);
+  int rc = scan2(ctx, SrcArray{lens}, SinkArray{offs}, P, &tot, st);
   if (rc) return rc;
   CK(hipMemcpyAsync(ctx->h_small, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   CK(hipStreamSynchronize(st));
@@ -1184,20 +1284,14 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *sta
   ctx->launches++;
   // bracket depths and matching
   if (B > 0) {
-    I2 *bval = (I2 *)ctx->buf(B_BVAL, B * sizeof(I2));
-    I2 *bincl = (I2 *)ctx->buf(B_BINCL, B * sizeof(I2));
     int32_t *depth = (int32_t *)ctx->buf(B_DEPTH, (B + 16) * sizeof(int32_t));
     int32_t *dl1 = (int32_t *)ctx->buf(B_DL1, ((B + 63) / 64 + 16) * sizeof(int32_t));
-    if (!bval || !bincl || !depth || !dl1) {
+    if (!depth || !dl1) {
       ctx->err = "scratch allocation failed";
       return ALPACC_ERR_HIP;
     }
-    bracket_vals<<<grid_for(B, 256), 256, 0, st>>>(br, B, bval);
-    ctx->launches++;
-    rc = scan_i2(ctx, bval, bincl, B, true, nullptr, st);
+    rc = scan2(ctx, SrcBracket{br}, SinkDepth{depth, dl1, flags, B}, B, nullptr, st);
     if (rc) return rc;
-    bracket_depth<<<grid_for(B, 256), 256, 0, st>>>(br, bincl, B, depth, dl1, flags);
-    ctx->launches++;
     Levels lv;
     rc = build_levels(ctx, depth, dl1, B, lv, st);
     if (rc) return rc;
@@ -1214,29 +1308,24 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *sta
     return ALPACC_ERR_CAPACITY;
   }
   if (N == 0) return ALPACC_ACCEPT;
-  I2 *pval = (I2 *)ctx->buf(B_PVAL, N * sizeof(I2));
-  I2 *pex = (I2 *)ctx->buf(B_PEX, N * sizeof(I2));
   int32_t *sx = (int32_t *)ctx->buf(B_SX, (N + 16) * sizeof(int32_t));
+  int32_t *rank = (int32_t *)ctx->buf(B_RANK, (N + 16) * sizeof(int32_t));
   int32_t *sl1 = (int32_t *)ctx->buf(B_SL1, ((N + 63) / 64 + 16) * sizeof(int32_t));
-  if (!pval || !pex || !sx || !sl1) {
+  if (!sx || !rank || !sl1) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
-  prod_vals<<<grid_for(N, 256), 256, 0, st>>>(a, prod, N, pval);
-  ctx->launches++;
   I2 ptot;
-  rc = scan_i2(ctx, pval, pex, N, false, &ptot, st);
+  rc = scan2(ctx, SrcProd{prod, ctx->d_arity, ctx->d_p2t}, SinkProd{sx, rank, sl1, N}, N, &ptot, st);
   if (rc) return rc;
   if ((uint64_t)(uint32_t)ptot.y != T) {
     ctx->err = "terminal productions do not match the token count";
     return ALPACC_ERR_ARG;
   }
-  split_x<<<grid_for(N, 256), 256, 0, st>>>(pex, N, sx, sl1);
-  ctx->launches++;
   Levels lv;
   rc = build_levels(ctx, sx, sl1, N, lv, st);
   if (rc) return rc;
-  parse_nodes<<<grid_for(N, 256), 256, 0, st>>>(a, prod, pex, N, lv, starts, ends, parent, node,
+  parse_nodes<<<grid_for(N, 256), 256, 0, st>>>(a, prod, rank, N, lv, starts, ends, parent, node,
                                                 span);
   ctx->launches++;
   CK(hipGetLastError());
