@@ -1,4 +1,6 @@
-"""Build the gfx950 HIP library in-tree (``alpacc_amd/libalpacc_hip.so``)."""
+"""Build the gfx950 HIP library in-tree (``alpacc_amd/libalpacc_hip.so``) and, for each bundled
+grammar, the stand-alone executable ``alpacc_amd/bin/alpacc_<grammar>`` (alpacc_main.c over the
+C ABI, tables from the header alpacc_amd.gen.emit_c writes into ``alpacc_amd/bin/``)."""
 from __future__ import annotations
 
 import os
@@ -16,6 +18,35 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Wno-unused-result"]
 
 
+GRAMMARS = ("json", "arithmetic", "sexp")
+BIN = os.path.join(HERE, "bin")
+CC = os.environ.get("CC", "gcc")
+
+
+def build_cli(grammars=GRAMMARS, verbose: bool = False) -> list:
+    """Emit each grammar's table header and link its executable against the HIP library."""
+    from .gen.emit_c import emit_header
+    from .gen.tables import generate
+    os.makedirs(BIN, exist_ok=True)
+    outs = []
+    for g in grammars:
+        header = os.path.join(BIN, f"{g}_tables.h")
+        text = emit_header(generate(os.path.join(HERE, "grammars", f"{g}.alp")), g)
+        with open(header, "w") as f:
+            f.write(text)
+        exe = os.path.join(BIN, f"alpacc_{g}")
+        cmd = [CC, "-O2", "-std=c11", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", BIN,
+               f'-DALPACC_TABLES_HEADER="{g}_tables.h"', f"-DALPACC_PREFIX={g}",
+               os.path.join(HERE, "csrc", "alpacc_main.c"), "-o", exe + ".tmp",
+               "-L", HERE, "-lalpacc_hip", "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(exe + ".tmp", exe)
+        outs.append(exe)
+    return outs
+
+
 def build(verbose: bool = False) -> str:
     tmp = OUT + ".tmp"
     cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), SRC, "-o", tmp]
@@ -28,3 +59,4 @@ def build(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print("\n".join(build_cli(verbose=True)))

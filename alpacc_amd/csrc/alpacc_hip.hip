@@ -643,33 +643,34 @@ __global__ void __launch_bounds__(SCAN_BLOCK) scan_reduce(Src src, uint64_t n, I
   }
 }
 
-// one block: exclusive scan of the block sums in place; the grand total goes to bsum[nb]
-__global__ void __launch_bounds__(SCAN_BLOCK) scan_blocks(I2 *bsum, uint64_t nb) {
-  __shared__ I2 wsum[SCAN_BLOCK / WAVE];
+// one block of 1024 threads: exclusive scan of the block sums in place (each thread owns a
+// contiguous run); the grand total goes to bsum[nb]
+constexpr int SCANB_THREADS = 1024;
+__global__ void __launch_bounds__(SCANB_THREADS) scan_blocks(I2 *bsum, uint64_t nb) {
+  __shared__ I2 wsum[SCANB_THREADS / WAVE];
   const int lane = threadIdx.x & (WAVE - 1);
   const int w = threadIdx.x / WAVE;
-  I2 carry{0, 0};
-  for (uint64_t off = 0; off < nb; off += SCAN_BLOCK) {
-    const uint64_t i = off + threadIdx.x;
-    const I2 v = i < nb ? bsum[i] : I2{0, 0};
-    const I2 incl = wave_incl2(v, lane);
-    if (lane == WAVE - 1) wsum[w] = incl;
-    __syncthreads();
-    I2 woff{0, 0}, tot{0, 0};
-    for (int k = 0; k < SCAN_BLOCK / WAVE; ++k) {
-      if (k < w) woff = add2(woff, wsum[k]);
-      tot = add2(tot, wsum[k]);
-    }
-    __syncthreads();
-    if (i < nb) {
-      I2 ex = add2(add2(carry, woff), incl);
-      ex.x -= v.x;
-      ex.y -= v.y;
-      bsum[i] = ex;
-    }
-    carry = add2(carry, tot);
+  const uint64_t per = (nb + SCANB_THREADS - 1) / SCANB_THREADS;
+  const uint64_t lo = min(nb, (uint64_t)threadIdx.x * per), hi = min(nb, lo + per);
+  I2 t{0, 0};
+  for (uint64_t i = lo; i < hi; ++i) t = add2(t, bsum[i]);
+  const I2 incl = wave_incl2(t, lane);
+  if (lane == WAVE - 1) wsum[w] = incl;
+  __syncthreads();
+  I2 run{0, 0}, tot{0, 0};
+  for (int k = 0; k < SCANB_THREADS / WAVE; ++k) {
+    if (k < w) run = add2(run, wsum[k]);
+    tot = add2(tot, wsum[k]);
   }
-  if (threadIdx.x == 0) bsum[nb] = carry;
+  run = add2(run, incl);
+  run.x -= t.x;
+  run.y -= t.y;
+  for (uint64_t i = lo; i < hi; ++i) {
+    const I2 v = bsum[i];
+    bsum[i] = run;
+    run = add2(run, v);
+  }
+  if (threadIdx.x == 0) bsum[nb] = tot;
 }
 
 template <class Src, class Sink>
@@ -781,55 +782,77 @@ __device__ int64_t prev_le_far(const Levels &lv, uint64_t chunk0, int32_t v) {
 
 __device__ __forceinline__ int hibit64(uint64_t m) { return 63 - __clzll((long long)m); }
 
+// Binary lifting over a wave-level sparse table of minima: lane l holds x_l; returns for every lane
+// the smallest p in [0, pos] such that x_p .. x_{pos-1} are all > v (p = pos when x_{pos-1} <= v).
+// All 64 lanes must call this together.
+__device__ __forceinline__ int lift_left(int32_t x, int pos, int32_t v) {
+  int32_t M[6];
+  M[0] = x;
+#pragma unroll
+  for (int k = 1; k < 6; ++k) {
+    const int32_t o = __shfl_down(M[k - 1], 1 << (k - 1), WAVE);
+    const int lane = threadIdx.x & (WAVE - 1);
+    M[k] = (lane + (1 << (k - 1)) < WAVE) ? min(M[k - 1], o) : M[k - 1];
+  }
+#pragma unroll
+  for (int k = 5; k >= 0; --k) {
+    const int step = 1 << k;
+    const int src = pos - step;
+    // M[k] at lane src covers x_src .. x_{src+step-1} = x_{pos-step} .. x_{pos-1}
+    const int32_t m = __shfl(M[k], src < 0 ? 0 : src, WAVE);
+    if (src >= 0 && m > v) pos = src;
+  }
+  // the steps add up to 63: from pos = 64 the loop stops at 1 without having looked at x_0
+  const int32_t x0 = __builtin_amdgcn_readlane(x, 0);
+  if (pos == 1 && x0 > v) pos = 0;
+  return pos;
+}
+
+// Minimum of the 64 chunks before `chunk` (lane l: chunk - 64 + l), the window the far search uses.
+__device__ __forceinline__ int32_t window_min(const Levels &lv, uint64_t chunk, int lane) {
+  if (lv.nlev < 2) return INT32_MAX;
+  const int64_t wc = (int64_t)chunk - WAVE + lane;
+  return wc >= 0 ? lv.L[1][wc] : INT32_MAX;
+}
+
 // Wave-cooperative previous smaller-or-equal for the 64 consecutive elements of chunk `chunk`
-// (lane l holds element chunk*64+l with value v; lanes past the end hold INT32_MAX). Steps:
-// the previous element; a ballot per unresolved lane inside the chunk; one coalesced load of the
-// 64 preceding chunk minima and a ballot per still unresolved lane, then a ballot over the chosen
-// chunk; a per-lane climb for what is left. All 64 lanes must call this together.
-__device__ int64_t prev_le_wave(const Levels &lv, uint64_t chunk, int lane, int32_t v, bool need) {
+// (lane l holds element chunk*64+l with value v; lanes past the end hold INT32_MAX; wv is
+// window_min of the chunk): binary lifting inside the chunk; then binary lifting over the window
+// picks the chunk holding the answer, and each distinct such chunk is loaded once (coalesced)
+// and searched by lifting again; a per-lane climb of the min hierarchy handles the rest.
+// All 64 lanes must call this together.
+__device__ int64_t prev_le_wave(const Levels &lv, uint64_t chunk, int lane, int32_t v, int32_t wv,
+                                bool need) {
   const int64_t cbase = (int64_t)(chunk * WAVE);
+  const int p0 = lift_left(v, lane, v);
   int64_t res = -1;
   bool done = !need;
-  const int32_t vp = __shfl_up(v, 1, WAVE);
-  if (!done && lane > 0 && vp <= v) {
-    res = cbase + lane - 1;
+  if (!done && p0 > 0) {
+    res = cbase + p0 - 1;
     done = true;
   }
-  uint64_t todo = __ballot(!done);
-  uint64_t it = todo;
-  while (it) {
-    const int t = __ffsll((long long)it) - 1;
-    it &= it - 1;
-    const int32_t vt = __builtin_amdgcn_readlane(v, t);
-    const uint64_t m = __ballot(v <= vt) & ((1ull << t) - 1ull);
-    if (lane == t && m) {
-      res = cbase + hibit64(m);
-      done = true;
-    }
-  }
-  todo = __ballot(!done);
-  if (!todo) return res;
+  if (__ballot(!done) == 0ull) return res;
   if (lv.nlev >= 2) {
-    const int64_t wc = (int64_t)chunk - WAVE + lane;
-    const int32_t wv = wc >= 0 ? lv.L[1][wc] : INT32_MAX;
-    it = todo;
-    while (it) {
-      const int t = __ffsll((long long)it) - 1;
-      it &= it - 1;
-      const int32_t vt = __builtin_amdgcn_readlane(v, t);
-      const uint64_t m = __ballot(wv <= vt);
-      if (m) {
-        const int64_t cb = (int64_t)chunk - WAVE + hibit64(m);
-        const int32_t x = lv.L[0][cb * WAVE + lane];
-        const uint64_t m2 = __ballot(x <= vt);
-        if (lane == t && m2) {
-          res = cb * WAVE + hibit64(m2);
-          done = true;
-        }
+    const int p1 = lift_left(wv, WAVE, v);
+    // target chunk of every unresolved lane (-1: not in the window)
+    int64_t cb = (!done && p1 > 0) ? (int64_t)chunk - WAVE + p1 - 1 : -1;
+    uint64_t pend = __ballot(cb >= 0);
+    while (pend) {
+      const int t = __ffsll((long long)pend) - 1;
+      const int64_t c = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cb & 0xFFFFFFFF), t) |
+                        ((int64_t)__builtin_amdgcn_readlane((int)(cb >> 32), t) << 32);
+      const int32_t x = lv.L[0][c * WAVE + lane];
+      const bool mine = cb == c;
+      const int p2 = lift_left(x, WAVE, mine ? v : INT32_MIN);
+      if (mine) {
+        res = c * WAVE + p2 - 1;
+        done = true;
+        cb = -1;
       }
+      pend = __ballot(cb >= 0);
     }
   }
-  if (!done) res = prev_le_far(lv, chunk, v);
+  if (!done) res = prev_le_far(lv, chunk > (uint64_t)WAVE ? chunk - WAVE : 0, v);
   return res;
 }
 
@@ -902,8 +925,9 @@ __global__ void bracket_match(const uint16_t *br, uint64_t B, Levels lv, uint32_
     const bool valid = i < B;
     const uint16_t b = valid ? br[i] : (uint16_t)0x8000u;
     const int32_t v = valid ? lv.L[0][i] : INT32_MAX;
+    const int32_t wv = window_min(lv, c, lane);
     const bool right = valid && !(b & 0x8000u);
-    const int64_t j = prev_le_wave(lv, c, lane, v, right);
+    const int64_t j = prev_le_wave(lv, c, lane, v, wv, right);
     if (right && (j < 0 || (br[j] & 0x7FFFu) != (b & 0x7FFFu))) atomicOr(flags, 1u);
   }
 }
@@ -920,19 +944,19 @@ __global__ void parse_nodes(ParseArgs a, const uint16_t *prod, const int32_t *ra
     const uint64_t i = c * WAVE + lane;
     const bool valid = i < N;
     const int32_t v = valid ? lv.L[0][i] : INT32_MAX;
-    const int64_t j = prev_le_wave(lv, c, lane, v, valid && i > 0);
-    if (!valid) continue;
-    parent[i] = i == 0 ? 0u : (uint32_t)j;
-    const uint16_t pr = prod[i];
-    const int32_t t = a.p2t[pr];
+    const int32_t wv = window_min(lv, c, lane);
+    // loads that do not depend on the search go out first
+    const uint16_t pr = valid ? prod[i] : (uint16_t)0;
+    const int32_t t = valid ? a.p2t[pr] : -1;
     uint2 sp = make_uint2(0u, 0u);
     if (t >= 0) {
       const uint32_t r = (uint32_t)rank[i];
-      node[i] = ALPACC_NODE_TERMINAL | (uint32_t)t;
       sp = make_uint2(starts[r], ends[r]);
-    } else {
-      node[i] = pr;
     }
+    const int64_t j = prev_le_wave(lv, c, lane, v, wv, valid && i > 0);
+    if (!valid) continue;
+    parent[i] = i == 0 ? 0u : (uint32_t)j;
+    node[i] = t >= 0 ? (ALPACC_NODE_TERMINAL | (uint32_t)t) : (uint32_t)pr;
     reinterpret_cast<uint2 *>(span)[i] = sp;
   }
 }
@@ -1082,7 +1106,7 @@ int scan2(alpacc_ctx *ctx, Src src, Sink sink, uint64_t n, I2 *total, hipStream_
   }
   if (n > 0) {
     scan_reduce<Src><<<(unsigned)nb, SCAN_BLOCK, 0, st>>>(src, n, bsum);
-    scan_blocks<<<1, SCAN_BLOCK, 0, st>>>(bsum, nb);
+    scan_blocks<<<1, SCANB_THREADS, 0, st>>>(bsum, nb);
     scan_down<Src, Sink><<<(unsigned)nb, SCAN_BLOCK, 0, st>>>(src, n, bsum, sink);
     ctx->launches += 3;
     CK(hipGetLastError());
