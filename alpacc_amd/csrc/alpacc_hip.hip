@@ -99,7 +99,33 @@ struct LexArgs {
   uint32_t *tile_counter;     // [1] dynamic count-tile index
   uint64_t *st_count;         // [num_ctiles] look-back words of the count scan
   uint32_t *result;           // [0] accept, [1] tokens, [2] timeout flag
+  // chunked mode (the reference's chunk loop, lexer.fut:144-172): `in` is the virtual string in
+  // which every chunk [j*chunk, min(j*chunk + chunk + 1, n_orig)) is laid end to end, so the byte
+  // shared by two neighbouring chunks appears twice.
+  uint32_t chunk;             // chunk size in bytes of the original string
+  uint32_t nchunks;           // K = ceil(n_orig / chunk)
+  uint64_t n_orig;
+  uint64_t *starts64;         // [cap] token starts as the reference computes them (i64 there)
+  uint64_t *ends64;
+  uint64_t *chunk_info;       // [K*4] per chunk: tokens through its end, kind, start, end of its
+                              // last element (the element that the next chunk's scatter clobbers)
 };
+
+// Span of a token of the chunked mode: the token ends at virtual position v, its start is the
+// virtual position w. lex_step (lexer.fut:91-142) of chunk j writes (start - j*chunk, i) with i the
+// chunk-local position, and the map at :131-133 of every later chunk j' >= j adds j'*chunk to the
+// start and 1 + j'*chunk to the end.
+__device__ __forceinline__ void chunk_span(const LexArgs &p, uint32_t w, uint32_t v, uint64_t &s,
+                                           uint64_t &e) {
+  const uint32_t cp1 = p.chunk + 1;
+  const uint32_t j = v / cp1;
+  const uint32_t i = v - j * cp1;
+  const uint64_t orig_w = (uint64_t)(w - w / cp1);
+  const uint64_t K = p.nchunks;
+  const uint64_t shift = (uint64_t)p.chunk * (((K - 1 + j) * (K - j)) / 2);
+  s = orig_w + shift - (uint64_t)j * p.chunk;
+  e = (uint64_t)i + (K - j) + shift;
+}
 
 __device__ __forceinline__ uint32_t comp_lookup(const uint16_t *tab, uint32_t e, uint32_t a,
                                                 uint32_t b) {
@@ -321,7 +347,11 @@ __device__ __forceinline__ uint64_t pack_count(uint32_t cnt, int32_t last, uint6
 
 // K3: walk the lexer DFA from the known lane start states, count token ends / find token starts,
 // scan them across the count tile (16 wave tiles) and across count tiles (decoupled look-back),
-// then walk again and write (kind, start, end) of every token.
+// then write (kind, start, end) of every token. CHUNKED: the input is the virtual string of the
+// chunked mode; the last element of every chunk but the last ends no token (i != n - 1 in
+// lex_step, lexer.fut:104-107), spans are shifted as lex_step does, and every chunk's last
+// element is recorded in chunk_info for lex_chunk_fixup.
+template <bool CHUNKED>
 __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   __shared__ uint16_t s_dfa[LEX_MAX_DFA];
   __shared__ uint8_t s_cls[256];
@@ -387,6 +417,13 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
     }
     s_end = s;
     lab_end = lab;
+  }
+  uint32_t lastm = 0;  // CHUNKED: chunk-last positions of this lane
+  if (CHUNKED && valid > 0) {
+    const uint32_t cp1 = p.chunk + 1;
+    const uint32_t r0 = (uint32_t)(base % cp1);
+    for (uint32_t j = p.chunk - r0; j < (uint32_t)valid; j += cp1) lastm |= 1u << j;
+    endm &= ~lastm;
   }
   const uint32_t cnt = __popc(endm);
   const int32_t last = startm ? (int32_t)(base + 31 - __clz(startm)) : -1;
@@ -479,7 +516,7 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
       const uint32_t o = k - r0;
       stk[o] = (lw >> (8 * (j & 3))) & 0xFFu;
       sts[o] = (uint32_t)cur;
-      ste[o] = (uint32_t)(base + j + 1);
+      ste[o] = (uint32_t)(base + j + (CHUNKED ? 0 : 1));
       ++k;
     }
     __builtin_amdgcn_wave_barrier();
@@ -489,12 +526,49 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
     for (uint32_t i = lane; i < m; i += WAVE) {
       const uint64_t g = (uint64_t)wout + r0 + i;
       p.kinds[g] = stk[i];
-      p.starts[g] = sts[i];
-      p.ends[g] = ste[i];
+      if (CHUNKED) {
+        uint64_t cs, ce;
+        chunk_span(p, sts[i], ste[i], cs, ce);
+        p.starts64[g] = cs;
+        p.ends64[g] = ce;
+      } else {
+        p.starts[g] = sts[i];
+        p.ends[g] = ste[i];
+      }
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  // ---- CHUNKED: the last element of every chunk that ends in this lane
+  if (CHUNKED) {
+    uint32_t lm = lastm;
+    while (lm) {
+      const int j = __ffs(lm) - 1;
+      lm &= lm - 1;
+      const uint32_t le = j == 31 ? 0xFFFFFFFFu : ((2u << j) - 1u);
+      const uint32_t sm = startm & le;
+      const uint32_t cur = sm ? (uint32_t)(base + 31 - __clz(sm)) : (uint32_t)start_in;
+      uint32_t lw;
+      switch (j >> 2) {
+        case 0: lw = labs[0]; break;
+        case 1: lw = labs[1]; break;
+        case 2: lw = labs[2]; break;
+        case 3: lw = labs[3]; break;
+        case 4: lw = labs[4]; break;
+        case 5: lw = labs[5]; break;
+        case 6: lw = labs[6]; break;
+        default: lw = labs[7]; break;
+      }
+      const uint32_t v = (uint32_t)(base + j);
+      uint64_t cs, ce;
+      chunk_span(p, cur, v, cs, ce);
+      uint64_t *ci = p.chunk_info + 4ull * (v / (p.chunk + 1));
+      ci[0] = (uint64_t)wout + lofs + __popc(endm & le);
+      ci[1] = (lw >> (8 * (j & 3))) & 0xFFu;
+      ci[2] = cs;
+      ci[3] = ce;
+    }
   }
   // ---- the lane holding the last byte appends the final token and decides acceptance
   if (valid > 0 && base + (uint64_t)valid == n) {
@@ -502,7 +576,24 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
     int32_t cur = start_in;
     if (startm) cur = (int32_t)(base + 31 - __clz(startm));
     uint32_t total = out;
-    if (lab_end != ign) {
+    if (CHUNKED) {
+      // the last chunk's last element, then the token lex_int_flag appends with the unshifted
+      // start and end n (lexer.fut:161-169)
+      const uint32_t v = (uint32_t)(n - 1);
+      uint64_t cs, ce;
+      chunk_span(p, (uint32_t)cur, v, cs, ce);
+      uint64_t *ci = p.chunk_info + 4ull * (p.nchunks - 1);
+      ci[0] = out;
+      ci[1] = lab_end;
+      ci[2] = cs;
+      ci[3] = ce;
+      if (lab_end != ign) {
+        p.kinds[out] = lab_end;
+        p.starts64[out] = (uint64_t)(cur - cur / (p.chunk + 1));
+        p.ends64[out] = p.n_orig;
+        total = out + 1;
+      }
+    } else if (lab_end != ign) {
       p.kinds[out] = lab_end;
       p.starts[out] = (uint32_t)cur;
       p.ends[out] = (uint32_t)n;
@@ -510,6 +601,32 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
     }
     p.result[1] = total;
     p.result[0] = p.state_accept[s_end] ? 1u : 0u;
+  }
+}
+
+// Chunked mode: in lex_step every element that ends no token is scattered to prev_size - 1
+// (lexer.fut:108-111, 131-132), the last token of the chunks before; with several such elements
+// the last one (the chunk's last element) is the one kept, and a later chunk that adds no token
+// overwrites the same slot again. Chunk j's element wins the slot iff chunk j + 1 moves the slot.
+__global__ void lex_chunk_fixup(LexArgs p) {
+  const uint32_t K = p.nchunks;
+  for (uint32_t j = 1 + blockIdx.x * blockDim.x + threadIdx.x; j < K; j += gridDim.x * blockDim.x) {
+    const uint64_t c = p.chunk_info[4ull * (j - 1)];
+    if (c == 0) continue;  // slot -1: outside the array, dropped by scatter
+    if (j + 1 < K && p.chunk_info[4ull * j] == c) continue;
+    p.kinds[c - 1] = (uint32_t)p.chunk_info[4ull * j + 1];
+    p.starts64[c - 1] = p.chunk_info[4ull * j + 2];
+    p.ends64[c - 1] = p.chunk_info[4ull * j + 3];
+  }
+}
+
+// Chunked mode: lay the chunks [j*chunk, min(j*chunk + chunk + 1, n)) end to end.
+__global__ void lex_chunk_expand(const uint8_t *in, uint32_t chunk, uint64_t nv, uint8_t *out) {
+  const uint32_t cp1 = chunk + 1;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nv;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t vv = (uint32_t)v;
+    out[v] = in[vv - vv / cp1];
   }
 }
 
@@ -933,9 +1050,13 @@ __global__ void bracket_match(const uint16_t *br, uint64_t B, Levels lv, uint32_
 }
 
 // parents (parser.fut:286-296) and terminal nodes (parser.fut:305-332). One wavefront per chunk.
+// S: span word (u32, or u64 for the chunked lexer's spans).
+// subst = 0 when the terminal productions do not pair up with the tokens: safe_zip then yields no
+// pairs (parser.fut:300-303) and every node stays a production node.
+template <typename S>
 __global__ void parse_nodes(ParseArgs a, const uint16_t *prod, const int32_t *rank, uint64_t N,
-                            Levels lv, const uint32_t *starts, const uint32_t *ends,
-                            uint32_t *parent, uint32_t *node, uint32_t *span) {
+                            Levels lv, const S *starts, const S *ends, uint32_t *parent,
+                            uint32_t *node, S *span, int subst) {
   const int lane = threadIdx.x & (WAVE - 1);
   const uint64_t nch = (N + WAVE - 1) / WAVE;
   const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
@@ -947,17 +1068,23 @@ __global__ void parse_nodes(ParseArgs a, const uint16_t *prod, const int32_t *ra
     const int32_t wv = window_min(lv, c, lane);
     // loads that do not depend on the search go out first
     const uint16_t pr = valid ? prod[i] : (uint16_t)0;
-    const int32_t t = valid ? a.p2t[pr] : -1;
-    uint2 sp = make_uint2(0u, 0u);
+    const int32_t t = valid && subst ? a.p2t[pr] : -1;
+    S s0 = 0, s1 = 0;
     if (t >= 0) {
       const uint32_t r = (uint32_t)rank[i];
-      sp = make_uint2(starts[r], ends[r]);
+      s0 = starts[r];
+      s1 = ends[r];
     }
     const int64_t j = prev_le_wave(lv, c, lane, v, wv, valid && i > 0);
     if (!valid) continue;
     parent[i] = i == 0 ? 0u : (uint32_t)j;
     node[i] = t >= 0 ? (ALPACC_NODE_TERMINAL | (uint32_t)t) : (uint32_t)pr;
-    reinterpret_cast<uint2 *>(span)[i] = sp;
+    if (sizeof(S) == 4) {
+      reinterpret_cast<uint2 *>(span)[i] = make_uint2((uint32_t)s0, (uint32_t)s1);
+    } else {
+      span[2 * i] = s0;
+      span[2 * i + 1] = s1;
+    }
   }
 }
 
@@ -1068,6 +1195,10 @@ enum BufId {
   B_SX,
   B_DL1,
   B_SL1,
+  B_VIRT,
+  B_CHUNK_INFO,
+  B_TOK_START64,
+  B_TOK_END64,
 };
 
 template <typename T>
@@ -1166,8 +1297,11 @@ int build_levels(alpacc_ctx *ctx, int32_t *base, int32_t *l1, uint64_t n, Levels
   return 0;
 }
 
+// The lexer. chunk == 0: unchunked (kinds/starts/ends as u32). chunk > 0: the reference's chunk
+// loop with chunks of `chunk` bytes (kinds, starts64/ends64).
 int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, uint32_t *starts,
-            uint32_t *ends, uint64_t *num_tokens, hipStream_t st) {
+            uint32_t *ends, uint64_t *num_tokens, hipStream_t st, uint64_t chunk = 0,
+            uint64_t *starts64 = nullptr, uint64_t *ends64 = nullptr) {
   if (!ctx->has_lexer) {
     ctx->err = "bundle has no lexer";
     return ALPACC_ERR_ARG;
@@ -1176,9 +1310,34 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
     *num_tokens = 0;
     return ALPACC_REJECT;
   }
+  const uint64_t n_orig = n;
+  uint64_t K = 1;
+  if (chunk > 0) {
+    if (chunk > n) chunk = n;  // one chunk: the same as any chunk size >= n
+    K = (n + chunk - 1) / chunk;
+    n = n + K - 1;  // the virtual string
+  }
   if (n >= (1ull << 31) - 2) {
     ctx->err = "documents must be shorter than 2^31 - 2 bytes";
     return ALPACC_ERR_ARG;
+  }
+  if (chunk > 0) {
+    uint8_t *virt = (uint8_t *)ctx->buf(B_VIRT, n + 16);
+    if (!virt) {
+      ctx->err = "scratch allocation failed";
+      return ALPACC_ERR_HIP;
+    }
+    lex_chunk_expand<<<grid_for(n, 256, 1u << 14), 256, 0, st>>>(d_in, (uint32_t)chunk, n, virt);
+    ctx->launches++;
+    d_in = virt;
+  }
+  uint64_t *chunk_info = nullptr;
+  if (chunk > 0) {
+    chunk_info = (uint64_t *)ctx->buf(B_CHUNK_INFO, K * 4 * sizeof(uint64_t));
+    if (!chunk_info) {
+      ctx->err = "scratch allocation failed";
+      return ALPACC_ERR_HIP;
+    }
   }
   const uint64_t tiles = (n + LEX_TILE - 1) / LEX_TILE;
   const uint64_t ctiles = (tiles + LEX_CT_WAVES - 1) / LEX_CT_WAVES;
@@ -1224,6 +1383,12 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   a.tile_counter = small;
   a.st_count = st_count;
   a.result = small + 4;
+  a.chunk = (uint32_t)chunk;
+  a.nchunks = (uint32_t)K;
+  a.n_orig = n_orig;
+  a.starts64 = starts64;
+  a.ends64 = ends64;
+  a.chunk_info = chunk_info;
   const uint64_t wpb = LEX_BLOCK / WAVE;
   uint64_t blocks = (tiles + wpb - 1) / wpb;
   blocks = std::min<uint64_t>(blocks, (uint64_t)std::max(1, ctx->cus) * 2);
@@ -1239,7 +1404,13 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   lex_tile_states<<<grid_for(tiles, 256), 256, 0, st>>>(a);
   ctx->prof_end(S_LEX_SCAN, st);
   ctx->prof_begin(S_LEX_EMIT, st);
-  lex_emit<<<(unsigned)ctiles, LEX_BLOCK, 0, st>>>(a);
+  if (chunk > 0) {
+    lex_emit<true><<<(unsigned)ctiles, LEX_BLOCK, 0, st>>>(a);
+    lex_chunk_fixup<<<grid_for(K, 256, 1024), 256, 0, st>>>(a);
+    ctx->launches++;
+  } else {
+    lex_emit<false><<<(unsigned)ctiles, LEX_BLOCK, 0, st>>>(a);
+  }
   ctx->prof_end(S_LEX_EMIT, st);
   ctx->launches += 7;
   CK(hipGetLastError());
@@ -1253,9 +1424,10 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   return ctx->h_small[0] ? ALPACC_ACCEPT : ALPACC_REJECT;
 }
 
-int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *starts,
-                     const uint32_t *ends, uint64_t T, uint32_t *parent, uint32_t *node,
-                     uint32_t *span, uint64_t cap, uint64_t *num_nodes, hipStream_t st) {
+template <typename S>
+int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, const S *ends,
+                     uint64_t T, uint32_t *parent, uint32_t *node, S *span, uint64_t cap,
+                     uint64_t *num_nodes, hipStream_t st) {
   if (!ctx->has_parser) {
     ctx->err = "bundle has no parser";
     return ALPACC_ERR_NO_PARSER;
@@ -1342,15 +1514,12 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const uint32_t *sta
   I2 ptot;
   rc = scan2(ctx, SrcProd{prod, ctx->d_arity, ctx->d_p2t}, SinkProd{sx, rank, sl1, N}, N, &ptot, st);
   if (rc) return rc;
-  if ((uint64_t)(uint32_t)ptot.y != T) {
-    ctx->err = "terminal productions do not match the token count";
-    return ALPACC_ERR_ARG;
-  }
+  const int subst = (uint64_t)(uint32_t)ptot.y == T ? 1 : 0;
   Levels lv;
   rc = build_levels(ctx, sx, sl1, N, lv, st);
   if (rc) return rc;
-  parse_nodes<<<grid_for(N, 256), 256, 0, st>>>(a, prod, rank, N, lv, starts, ends, parent, node,
-                                                span);
+  parse_nodes<S><<<grid_for(N, 256), 256, 0, st>>>(a, prod, rank, N, lv, starts, ends, parent,
+                                                   node, span, subst);
   ctx->launches++;
   CK(hipGetLastError());
   ctx->prof_end(S_PARSE, st);
@@ -1533,8 +1702,8 @@ int alpacc_parse_tokens(alpacc_ctx *ctx, const uint32_t *d_kinds, const uint32_t
   ctx->launches = 0;
   ctx->err.clear();
   (void)hipSetDevice(ctx->device);
-  const int rc = run_parse_tokens(ctx, d_kinds, d_starts, d_ends, num_tokens, d_parent, d_node,
-                                  d_span, cap, num_nodes, (hipStream_t)stream);
+  const int rc = run_parse_tokens<uint32_t>(ctx, d_kinds, d_starts, d_ends, num_tokens, d_parent,
+                                            d_node, d_span, cap, num_nodes, (hipStream_t)stream);
   if (rc != ALPACC_ACCEPT) ctx->prof_drop(S_PARSE);
   if (ctx->prof) ctx->prof_collect();
   return rc;
@@ -1568,7 +1737,65 @@ int alpacc_parse(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_p
     return rc;
   }
   const int lex_launches = ctx->launches;
-  rc = run_parse_tokens(ctx, kinds, starts, ends, T, d_parent, d_node, d_span, cap, num_nodes, st);
+  rc = run_parse_tokens<uint32_t>(ctx, kinds, starts, ends, T, d_parent, d_node, d_span, cap,
+                                  num_nodes, st);
+  ctx->launches += lex_launches;
+  if (rc != ALPACC_ACCEPT) ctx->prof_drop(S_PARSE);
+  if (ctx->prof) ctx->prof_collect();
+  return rc;
+}
+
+int alpacc_lex_chunked(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint64_t chunk_bytes,
+                       uint32_t *d_kinds, uint64_t *d_starts, uint64_t *d_ends, uint64_t cap,
+                       uint64_t *num_tokens, void *stream) {
+  if (!ctx || !num_tokens || chunk_bytes == 0 ||
+      (n > 0 && (!d_in || !d_kinds || !d_starts || !d_ends)))
+    return ALPACC_ERR_ARG;
+  const uint64_t c = std::min(chunk_bytes, std::max<uint64_t>(n, 1));
+  if (cap < n + (n + c - 1) / c) {
+    ctx->err = "token capacity must be at least n + ceil(n / chunk_bytes)";
+    return ALPACC_ERR_CAPACITY;
+  }
+  ctx->launches = 0;
+  ctx->err.clear();
+  (void)hipSetDevice(ctx->device);
+  const int rc = run_lex(ctx, d_in, n, d_kinds, nullptr, nullptr, num_tokens, (hipStream_t)stream,
+                         chunk_bytes, d_starts, d_ends);
+  if (ctx->prof) ctx->prof_collect();
+  return rc;
+}
+
+int alpacc_parse_chunked(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint64_t chunk_bytes,
+                         uint32_t *d_parent, uint32_t *d_node, uint64_t *d_span, uint64_t cap,
+                         uint64_t *num_nodes, void *stream) {
+  if (!ctx || !num_nodes || chunk_bytes == 0) return ALPACC_ERR_ARG;
+  ctx->launches = 0;
+  ctx->err.clear();
+  *num_nodes = 0;
+  if (!ctx->has_parser) {
+    ctx->err = "bundle has no parser";
+    return ALPACC_ERR_NO_PARSER;
+  }
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  const uint64_t c = std::min(chunk_bytes, std::max<uint64_t>(n, 1));
+  const uint64_t tcap = n + (n + c - 1) / c + 1;
+  uint32_t *kinds = (uint32_t *)ctx->buf(B_TOK_KIND, tcap * 4);
+  uint64_t *starts = (uint64_t *)ctx->buf(B_TOK_START64, tcap * 8);
+  uint64_t *ends = (uint64_t *)ctx->buf(B_TOK_END64, tcap * 8);
+  if (!kinds || !starts || !ends) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  uint64_t T = 0;
+  int rc = run_lex(ctx, d_in, n, kinds, nullptr, nullptr, &T, st, chunk_bytes, starts, ends);
+  if (rc != ALPACC_ACCEPT) {
+    if (ctx->prof) ctx->prof_collect();
+    return rc;
+  }
+  const int lex_launches = ctx->launches;
+  rc = run_parse_tokens<uint64_t>(ctx, kinds, starts, ends, T, d_parent, d_node, d_span, cap,
+                                  num_nodes, st);
   ctx->launches += lex_launches;
   if (rc != ALPACC_ACCEPT) ctx->prof_drop(S_PARSE);
   if (ctx->prof) ctx->prof_collect();

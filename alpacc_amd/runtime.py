@@ -58,7 +58,11 @@ EXPORTS = ("alpacc_create", "alpacc_create_error", "alpacc_destroy", "alpacc_las
            "alpacc_copy_to_device", "alpacc_copy_to_host", "alpacc_device_memset",
            "alpacc_stream_create", "alpacc_stream_destroy", "alpacc_stream_sync",
            "alpacc_event_create", "alpacc_event_destroy", "alpacc_event_record",
-           "alpacc_event_elapsed_ms", "alpacc_profile_enable", "alpacc_profile_read")
+           "alpacc_event_elapsed_ms", "alpacc_profile_enable", "alpacc_profile_read",
+           "alpacc_lex_chunked", "alpacc_parse_chunked")
+
+# the chunk size of the reference's generated entries (Generator.hs:24-76: lexer.lex_int 16777216)
+REFERENCE_CHUNK = 16777216
 
 _lib = None
 
@@ -84,6 +88,8 @@ def load_library():
     L.alpacc_lex.argtypes = [vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
     L.alpacc_parse.argtypes = [vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
     L.alpacc_parse_tokens.argtypes = [vp, vp, vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
+    L.alpacc_lex_chunked.argtypes = [vp, vp, u64, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
+    L.alpacc_parse_chunked.argtypes = [vp, vp, u64, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
     L.alpacc_device_count.restype = ctypes.c_int
     L.alpacc_set_device.argtypes = [ctypes.c_int]
     L.alpacc_device_alloc.argtypes = [u64]
@@ -296,9 +302,27 @@ class Context:
             raise self._err(rc)
         return rc, nn.value
 
+    def lex_chunked_device(self, d_in: int, n: int, chunk: int, kinds: int, starts: int, ends: int,
+                           cap: int, stream=None) -> tuple[int, int]:
+        """``alpacc_lex_chunked`` on device pointers; returns (status, token count)."""
+        nt = ctypes.c_uint64(0)
+        rc = self.lib.alpacc_lex_chunked(self.handle, ctypes.c_void_p(d_in), n, chunk,
+                                         ctypes.c_void_p(kinds), ctypes.c_void_p(starts),
+                                         ctypes.c_void_p(ends), cap, ctypes.byref(nt), stream)
+        if rc < 0:
+            raise self._err(rc)
+        return rc, nt.value
+
     # -- reference entries ------------------------------------------------------------------
-    def lex_int(self, data):
-        """``lex_int``: None when rejected, else (kinds, starts, ends) as uint32 numpy arrays."""
+    def lex_int(self, data, chunk_size: int | None = None):
+        """``lex_int``: None when rejected, else (kinds, starts, ends) numpy arrays.
+
+        ``chunk_size=None``: unchunked semantics, uint32 spans. ``chunk_size=c``: the reference's
+        chunk loop over chunks of c bytes (``REFERENCE_CHUNK`` is what its entries use), uint64
+        spans.
+        """
+        if chunk_size is not None:
+            return self._lex_int_chunked(data, int(chunk_size))
         d_in, n = self.upload_doc(data)
         cap = max(n, 1)
         kb = self.buffer("kinds", 4 * cap)
@@ -310,18 +334,35 @@ class Context:
         return (kb.download(np.uint32, nt, self.stream), sb.download(np.uint32, nt, self.stream),
                 eb.download(np.uint32, nt, self.stream))
 
-    def lex(self, data):
-        r = self.lex_int(data)
+    def _lex_int_chunked(self, data, chunk: int):
+        d_in, n = self.upload_doc(data)
+        c = min(chunk, max(n, 1))
+        cap = n + (n + c - 1) // c + 1
+        kb = self.buffer("kinds", 4 * cap)
+        sb = self.buffer("starts64", 8 * cap)
+        eb = self.buffer("ends64", 8 * cap)
+        rc, nt = self.lex_chunked_device(d_in.ptr, n, chunk, kb.ptr, sb.ptr, eb.ptr, cap, self.stream)
+        if rc != ACCEPT:
+            return None
+        return (kb.download(np.uint32, nt, self.stream), sb.download(np.uint64, nt, self.stream),
+                eb.download(np.uint64, nt, self.stream))
+
+    def lex(self, data, chunk_size: int | None = None):
+        r = self.lex_int(data, chunk_size)
         if r is None:
             return None
         names = self.bundle.terminal_names
         return [(names[k], (int(s), int(e))) for k, s, e in zip(*r)]
 
-    def parse_int(self, data, cap: int | None = None):
-        """``parse_int``: None when rejected, else (parent, node, span[N,2]) uint32 numpy arrays.
+    def parse_int(self, data, cap: int | None = None, chunk_size: int | None = None):
+        """``parse_int``: None when rejected, else (parent, node, span[N,2]) numpy arrays.
 
         ``node`` has bit 31 set for terminal nodes (low bits = terminal id), else a production id.
+        ``chunk_size=None``: unchunked lexer, uint32 spans; ``chunk_size=c``: the reference's
+        chunked lexer (``REFERENCE_CHUNK`` for its generated ``parse_int``), uint64 spans.
         """
+        if chunk_size is not None:
+            return self._parse_int_chunked(data, cap, int(chunk_size))
         d_in, n = self.upload_doc(data)
         if cap is None:
             cap = max(16, 4 * n + 16)
@@ -339,8 +380,33 @@ class Context:
         return (pb.download(np.uint32, nn, self.stream), nb.download(np.uint32, nn, self.stream),
                 spb.download(np.uint32, 2 * nn, self.stream).reshape(-1, 2))
 
-    def parse(self, data):
-        r = self.parse_int(data)
+    def _parse_int_chunked(self, data, cap, chunk: int):
+        d_in, n = self.upload_doc(data)
+        if cap is None:
+            cap = max(16, 4 * n + 16)
+        while True:
+            pb = self.buffer("parent", 4 * cap)
+            nb = self.buffer("node", 4 * cap)
+            spb = self.buffer("span64", 16 * cap)
+            nn = ctypes.c_uint64(0)
+            rc = self.lib.alpacc_parse_chunked(self.handle, ctypes.c_void_p(d_in.ptr), n, chunk,
+                                               ctypes.c_void_p(pb.ptr), ctypes.c_void_p(nb.ptr),
+                                               ctypes.c_void_p(spb.ptr), cap, ctypes.byref(nn),
+                                               self.stream)
+            if rc == -3:
+                cap = nn.value
+                continue
+            if rc < 0:
+                raise self._err(rc)
+            break
+        if rc != ACCEPT:
+            return None
+        nn = nn.value
+        return (pb.download(np.uint32, nn, self.stream), nb.download(np.uint32, nn, self.stream),
+                spb.download(np.uint64, 2 * nn, self.stream).reshape(-1, 2))
+
+    def parse(self, data, chunk_size: int | None = None):
+        r = self.parse_int(data, chunk_size=chunk_size)
         if r is None:
             return None
         tnames = self.bundle.terminal_names

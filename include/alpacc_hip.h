@@ -100,6 +100,27 @@ int alpacc_parse_tokens(alpacc_ctx *ctx, const uint32_t *d_kinds, const uint32_t
                         uint32_t *d_node, uint32_t *d_span, uint64_t cap, uint64_t *num_nodes,
                         void *stream);
 
+/* The reference's chunk size: its generated entries call lexer.lex_int 16777216
+ * (src/Alpacc/Generator/Futhark/Generator.hs:24-76). */
+#define ALPACC_REFERENCE_CHUNK 16777216ull
+
+/* lex_int with the reference's chunk loop (lex_int_flag, futhark/lexer.fut:144-172) over chunks of
+ * chunk_bytes: the byte shared by neighbouring chunks is run through the DFA twice, the element that
+ * ends each chunk clobbers the last token of the chunks before it, and spans carry the shifts the
+ * loop applies; starts/ends are therefore 64-bit (i64 in the reference). Equals alpacc_lex (as
+ * 64-bit spans) whenever n <= chunk_bytes. Outputs: device kinds u32 [cap], starts/ends u64 [cap],
+ * cap >= n + ceil(n / chunk_bytes). */
+int alpacc_lex_chunked(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint64_t chunk_bytes,
+                       uint32_t *d_kinds, uint64_t *d_starts, uint64_t *d_ends, uint64_t cap,
+                       uint64_t *num_tokens, void *stream);
+
+/* parse_int of the reference's lexer+parser artifact exactly as generated (Generator.hs:24-44:
+ * lexer.lex_int 16777216 then parser.parse_int) for chunk_bytes = ALPACC_REFERENCE_CHUNK: the
+ * chunked lexer followed by the parser. span is u64 [2*cap]. */
+int alpacc_parse_chunked(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint64_t chunk_bytes,
+                         uint32_t *d_parent, uint32_t *d_node, uint64_t *d_span, uint64_t cap,
+                         uint64_t *num_nodes, void *stream);
+
 /* Number of kernels the last call on `ctx` launched. */
 int alpacc_last_launch_count(alpacc_ctx *ctx);
 
