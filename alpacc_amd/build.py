@@ -25,8 +25,11 @@ CC = os.environ.get("CC", "gcc")
 
 def build_cli(grammars=GRAMMARS, verbose: bool = False) -> list:
     """Emit each grammar's table header and link its executable against the HIP library."""
-    from .gen.emit_c import emit_header
-    from .gen.tables import generate
+    import sys
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    from alpacc_amd.gen.emit_c import emit_header
+    from alpacc_amd.gen.tables import generate
     os.makedirs(BIN, exist_ok=True)
     outs = []
     for g in grammars:

@@ -109,7 +109,56 @@ struct LexArgs {
   uint64_t *ends64;
   uint64_t *chunk_info;       // [K*4] per chunk: tokens through its end, kind, start, end of its
                               // last element (the element that the next chunk's scatter clobbers)
+  // batched mode: many documents in one pass. Every document starts a new wave tile (tiles are
+  // numbered through the documents in order), the DFA starts afresh and token positions restart
+  // at 0 in every document, and document d's tokens go to slots [tok_base, tok_base + len + 1).
+  const struct LexDoc *docs;  // [ndocs]
+  const uint32_t *tile_doc;   // [num_tiles] document of every wave tile
+  uint32_t ndocs;
+  uint32_t *doc_res;          // [2*ndocs] accept flag, token count
 };
+
+struct LexDoc {
+  uint64_t offset;    // first byte in `in`
+  uint64_t len;       // bytes
+  uint64_t tok_base;  // first token slot
+  uint32_t tile_base; // first wave tile
+  uint32_t ntiles;
+};
+
+// The bytes a wave tile covers: document base pointer and length, the tile's index inside its
+// document, whether it is the document's first tile.
+struct TileView {
+  const uint8_t *ptr;
+  uint64_t n;
+  uint64_t tok_base;
+  uint32_t doc;
+  uint32_t ltile;
+  bool aligned;
+};
+
+template <bool BATCH>
+__device__ __forceinline__ TileView tile_view(const LexArgs &p, uint32_t tile) {
+  TileView v;
+  if (!BATCH) {
+    v.ptr = p.in;
+    v.n = p.n;
+    v.tok_base = 0;
+    v.doc = 0;
+    v.ltile = tile;
+    v.aligned = p.aligned != 0;
+  } else {
+    const uint32_t d = p.tile_doc[tile];
+    const LexDoc doc = p.docs[d];
+    v.ptr = p.in + doc.offset;
+    v.n = doc.len;
+    v.tok_base = doc.tok_base;
+    v.doc = d;
+    v.ltile = tile - doc.tile_base;
+    v.aligned = ((uintptr_t)v.ptr & 15) == 0;
+  }
+  return v;
+}
 
 // Span of a token of the chunked mode: the token ends at virtual position v, its start is the
 // virtual position w. lex_step (lexer.fut:91-142) of chunk j writes (start - j*chunk, i) with i the
@@ -135,11 +184,11 @@ __device__ __forceinline__ uint32_t comp_lookup(const uint16_t *tab, uint32_t e,
 #define LEX_CLS(j) ((cls[(j) >> 2] >> (8 * ((j)&3))) & 0xFFu)
 
 // bytes of one lane -> byte classes, four per register
-__device__ __forceinline__ void load_classes(const LexArgs &p, const uint8_t *s_cls, uint64_t base,
-                                             int valid, uint32_t (&cls)[LEX_IPT / 4]) {
-  if (valid == LEX_IPT && p.aligned) {
-    const uint4 a = *reinterpret_cast<const uint4 *>(p.in + base);
-    const uint4 b = *reinterpret_cast<const uint4 *>(p.in + base + 16);
+__device__ __forceinline__ void load_classes(const uint8_t *in, bool aligned, const uint8_t *s_cls,
+                                             uint64_t base, int valid, uint32_t (&cls)[LEX_IPT / 4]) {
+  if (valid == LEX_IPT && aligned) {
+    const uint4 a = *reinterpret_cast<const uint4 *>(in + base);
+    const uint4 b = *reinterpret_cast<const uint4 *>(in + base + 16);
     const uint32_t wds[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -154,7 +203,7 @@ __device__ __forceinline__ void load_classes(const LexArgs &p, const uint8_t *s_
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int j = q * 4 + r;
-        if (j < valid) v |= (uint32_t)s_cls[p.in[base + j]] << (8 * r);
+        if (j < valid) v |= (uint32_t)s_cls[in[base + j]] << (8 * r);
       }
       cls[q] = v;
     }
@@ -164,7 +213,7 @@ __device__ __forceinline__ void load_classes(const LexArgs &p, const uint8_t *s_
 // K1: per wave tile, the endomorphism of every lane's bytes (serial walk through the
 // (endo x class) table in LDS) and their wave-exclusive composition (Blelloch up/down sweep with
 // the (endo x endo) table, 126 lookups per wave). Persistent, grid-stride over wave tiles.
-template <bool COMP_IN_LDS>
+template <bool COMP_IN_LDS, bool BATCH>
 __global__ void __launch_bounds__(LEX_BLOCK) lex_reduce(LexArgs p) {
   extern __shared__ __align__(16) uint8_t smem[];
   const uint32_t e = p.e, C = p.C;
@@ -179,13 +228,13 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_reduce(LexArgs p) {
   const uint16_t *comp_tab = COMP_IN_LDS ? s_comp : p.endo_comp;
   const int lane = threadIdx.x & (WAVE - 1);
   const uint32_t wpb = blockDim.x / WAVE;
-  const uint64_t n = p.n;
   for (uint32_t tile = blockIdx.x * wpb + threadIdx.x / WAVE; tile < p.num_tiles;
        tile += gridDim.x * wpb) {
-    const uint64_t base = (uint64_t)tile * LEX_TILE + (uint64_t)lane * LEX_IPT;
-    const int valid = base >= n ? 0 : (int)min((uint64_t)LEX_IPT, n - base);
+    const TileView tv = tile_view<BATCH>(p, tile);
+    const uint64_t base = (uint64_t)tv.ltile * LEX_TILE + (uint64_t)lane * LEX_IPT;
+    const int valid = base >= tv.n ? 0 : (int)min((uint64_t)LEX_IPT, tv.n - base);
     uint32_t cls[LEX_IPT / 4];
-    load_classes(p, s_cls, base, valid, cls);
+    load_classes(tv.ptr, tv.aligned, s_cls, base, valid, cls);
     uint32_t v = 0;
 #pragma unroll
     for (int j = 0; j < LEX_IPT; ++j)
@@ -304,6 +353,65 @@ __global__ void lex_tile_states(LexArgs p) {
   p.tile_state[t] = p.endo_apply[pre * p.m1 + p.init];
 }
 
+// K2 of the batched mode: one block per document; exclusive composition of the document's wave
+// tiles (Blelloch in LDS, chunks with a carry), then the DFA state at the start of every tile.
+__global__ void __launch_bounds__(LEX_SCAN_BLOCK) lex_scan_docs(LexArgs p) {
+  __shared__ uint16_t s[LEX_SCAN_BLOCK];
+  __shared__ uint16_t carry_s;
+  const LexDoc doc = p.docs[blockIdx.x];
+  const uint32_t i = threadIdx.x;
+  if (i == 0) carry_s = 0;
+  for (uint32_t off = 0; off < doc.ntiles; off += LEX_SCAN_BLOCK) {
+    const uint32_t t = doc.tile_base + off + i;
+    const bool in = off + i < doc.ntiles;
+    s[i] = in ? p.tile_agg[t] : 0;
+    __syncthreads();
+    for (uint32_t d = 1; d < LEX_SCAN_BLOCK; d <<= 1) {
+      uint32_t nv = 0;
+      const bool act = ((i + 1) & (2 * d - 1)) == 0;
+      if (act) nv = comp_lookup(p.endo_comp, p.e, s[i - d], s[i]);
+      __syncthreads();
+      if (act) s[i] = (uint16_t)nv;
+      __syncthreads();
+    }
+    uint32_t chunk_total = 0;
+    if (i == LEX_SCAN_BLOCK - 1) {
+      chunk_total = s[i];
+      s[i] = 0;
+    }
+    __syncthreads();
+    for (uint32_t d = LEX_SCAN_BLOCK / 2; d >= 1; d >>= 1) {
+      const bool right = ((i + 1) & (2 * d - 1)) == 0;
+      uint32_t l = 0, r = 0;
+      if (right) {
+        l = s[i - d];
+        r = s[i];
+      }
+      __syncthreads();
+      if (right) {
+        s[i - d] = (uint16_t)r;
+        s[i] = (uint16_t)comp_lookup(p.endo_comp, p.e, r, l);
+      }
+      __syncthreads();
+    }
+    const uint32_t carry = carry_s;
+    if (in) {
+      const uint32_t pre = comp_lookup(p.endo_comp, p.e, carry, s[i]);
+      p.tile_state[t] = p.endo_apply[pre * p.m1 + p.init];
+    }
+    __syncthreads();
+    if (i == LEX_SCAN_BLOCK - 1) carry_s = (uint16_t)comp_lookup(p.endo_comp, p.e, carry, chunk_total);
+    __syncthreads();
+  }
+}
+
+// Batched mode: the document of every wave tile.
+__global__ void lex_tile_docs(LexArgs p) {
+  const LexDoc doc = p.docs[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < doc.ntiles; i += blockDim.x)
+    const_cast<uint32_t *>(p.tile_doc)[doc.tile_base + i] = blockIdx.x;
+}
+
 // Exclusive (token count, last token start) of a count tile: decoupled look-back; the two
 // components combine by + and max, so the window reduces with plain wave reductions.
 __device__ void lookback_count(const LexArgs &p, uint32_t tile, int lane, uint32_t &cnt,
@@ -351,12 +459,13 @@ __device__ __forceinline__ uint64_t pack_count(uint32_t cnt, int32_t last, uint6
 // chunked mode; the last element of every chunk but the last ends no token (i != n - 1 in
 // lex_step, lexer.fut:104-107), spans are shifted as lex_step does, and every chunk's last
 // element is recorded in chunk_info for lex_chunk_fixup.
-template <bool CHUNKED>
+template <bool CHUNKED, bool BATCH>
 __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   __shared__ uint16_t s_dfa[LEX_MAX_DFA];
   __shared__ uint8_t s_cls[256];
   __shared__ uint32_t s_wcnt[LEX_CT_WAVES];
   __shared__ int32_t s_wlast[LEX_CT_WAVES];
+  __shared__ uint32_t s_wreset[LEX_CT_WAVES];
   __shared__ uint32_t s_ctile;
   __shared__ uint32_t s_stage[LEX_CT_WAVES][3][LEX_STAGE];
   const uint32_t C = p.C;
@@ -368,13 +477,25 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   const int lane = threadIdx.x & (WAVE - 1);
   const int w = threadIdx.x / WAVE;
   const uint32_t tile = ct * LEX_CT_WAVES + w;
-  const uint64_t n = p.n;
   const uint32_t ign = p.ignore < 0 ? 0xFFFFFFFFu : (uint32_t)p.ignore;
-  const uint64_t tbase = (uint64_t)tile * LEX_TILE;
+  // positions are relative to the tile's document
+  TileView tv;
+  if (tile < p.num_tiles) {
+    tv = tile_view<BATCH>(p, tile);
+  } else {
+    tv.ptr = p.in;
+    tv.n = 0;
+    tv.tok_base = 0;
+    tv.doc = 0;
+    tv.ltile = 0;
+    tv.aligned = false;
+  }
+  const uint64_t n = tv.n;
+  const uint64_t tbase = (uint64_t)tv.ltile * LEX_TILE;
   const uint64_t base = tbase + (uint64_t)lane * LEX_IPT;
   const int valid = (tile >= p.num_tiles || base >= n) ? 0 : (int)min((uint64_t)LEX_IPT, n - base);
   uint32_t cls[LEX_IPT / 4];
-  load_classes(p, s_cls, base, valid, cls);
+  load_classes(tv.ptr, tv.aligned, s_cls, base, valid, cls);
   uint32_t s_lane = 0;
   if (tile < p.num_tiles) {
     const uint32_t s_tile = p.tile_state[tile];
@@ -384,7 +505,7 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   uint32_t c_next = __shfl_down(LEX_CLS(0), 1, WAVE);
   if (lane == WAVE - 1) {
     const uint64_t nb = tbase + LEX_TILE;
-    c_next = nb < n ? s_cls[p.in[nb]] : 0u;
+    c_next = nb < n ? s_cls[tv.ptr[nb]] : 0u;
   }
 
   // ---- one walk: token-end mask, token-start mask and the label after every byte
@@ -443,43 +564,68 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   if (lane == WAVE - 1) {
     s_wcnt[w] = cincl;
     s_wlast[w] = lincl;
+    // a wave tile that starts a document restarts the count and the last start
+    s_wreset[w] = (tile < p.num_tiles && tv.ltile == 0) ? 1u : 0u;
   }
   __syncthreads();
   if (w == 0) {
-    uint32_t wc = lane < LEX_CT_WAVES ? s_wcnt[lane] : 0u;
-    int32_t wl = lane < LEX_CT_WAVES ? s_wlast[lane] : -1;
+    // segmented scan of the 16 wave totals (a reset keeps only what follows it)
+    const uint32_t wc = lane < LEX_CT_WAVES ? s_wcnt[lane] : 0u;
+    const int32_t wl = lane < LEX_CT_WAVES ? s_wlast[lane] : -1;
+    const bool wr = lane < LEX_CT_WAVES ? s_wreset[lane] != 0 : false;
     uint32_t wci = wc;
     int32_t wli = wl;
+    bool wfi = wr;
 #pragma unroll
     for (int d = 1; d < LEX_CT_WAVES; d <<= 1) {
       const uint32_t oc = __shfl_up(wci, d, WAVE);
       const int32_t ol = __shfl_up(wli, d, WAVE);
-      if (lane >= d) {
+      const bool of = __shfl_up((int)wfi, d, WAVE) != 0;
+      if (lane >= d && !wfi) {
         wci += oc;
         wli = max(wli, ol);
+        wfi = of;
       }
     }
     const uint32_t ctot = __shfl(wci, LEX_CT_WAVES - 1, WAVE);
     const int32_t ltot = __shfl(wli, LEX_CT_WAVES - 1, WAVE);
-    if (lane == 0) st_relaxed64(p.st_count + ct, pack_count(ctot, ltot, ct == 0 ? CT_INC : CT_AGG));
+    const bool ftot = __shfl((int)wfi, LEX_CT_WAVES - 1, WAVE) != 0;
+    // a count tile holding a document start already has its (post-reset) inclusive value
+    if (lane == 0) st_relaxed64(p.st_count + ct, pack_count(ctot, ltot, ftot ? CT_INC : CT_AGG));
     uint32_t cpre = 0;
     int32_t lpre = -1;
-    if (ct > 0) {
+    const bool need_lb = ct > 0 && __shfl((int)wr, 0, WAVE) == 0;
+    if (need_lb) {
       lookback_count(p, ct, lane, cpre, lpre);
-      if (lane == 0) st_relaxed64(p.st_count + ct, pack_count(cpre + ctot, max(lpre, ltot), CT_INC));
+      if (lane == 0 && !ftot)
+        st_relaxed64(p.st_count + ct, pack_count(cpre + ctot, max(lpre, ltot), CT_INC));
     }
-    int32_t wlex = __shfl_up(wli, 1, WAVE);
-    const uint32_t wcex = wci - wc;
-    if (lane == 0) wlex = -1;
+    // exclusive value of every wave: the inclusive value of the wave before (or nothing after a
+    // reset), plus the look-back prefix when no reset precedes it in this count tile
+    uint32_t xc = __shfl_up(wci, 1, WAVE);
+    int32_t xl = __shfl_up(wli, 1, WAVE);
+    bool xf = __shfl_up((int)wfi, 1, WAVE) != 0;
+    if (lane == 0) {
+      xc = 0;
+      xl = -1;
+      xf = false;
+    }
+    if (wr) {
+      xc = 0;
+      xl = -1;
+    } else if (!xf) {
+      xc += cpre;
+      xl = max(xl, lpre);
+    }
     if (lane < LEX_CT_WAVES) {
-      s_wcnt[lane] = cpre + wcex;
-      s_wlast[lane] = max(lpre, wlex);
+      s_wcnt[lane] = xc;
+      s_wlast[lane] = xl;
     }
   }
   __syncthreads();
 
   // ---- write tokens: stage in LDS by wave-local index, copy out coalesced, in rounds
-  const uint32_t wout = s_wcnt[w];                  // global index of the wave's first token
+  const uint32_t wout = (uint32_t)tv.tok_base + s_wcnt[w];  // slot of the wave's first token
   const uint32_t wtotal = __shfl(cincl, WAVE - 1, WAVE);
   const uint32_t lofs = cincl - cnt;                // wave-local index of this lane's first token
   int32_t lexcl = __shfl_up(lincl, 1, WAVE);
@@ -599,8 +745,13 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
       p.ends[out] = (uint32_t)n;
       total = out + 1;
     }
-    p.result[1] = total;
-    p.result[0] = p.state_accept[s_end] ? 1u : 0u;
+    if (BATCH) {
+      p.doc_res[2 * tv.doc] = p.state_accept[s_end] ? 1u : 0u;
+      p.doc_res[2 * tv.doc + 1] = total - (uint32_t)tv.tok_base;
+    } else {
+      p.result[1] = total;
+      p.result[0] = p.state_accept[s_end] ? 1u : 0u;
+    }
   }
 }
 
@@ -634,194 +785,6 @@ __global__ void clear_u32(uint32_t *p, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = 0;
-}
-
-// ----------------------------------------------------------------------------------------------
-// device-wide exclusive/inclusive scans of int2 (two independent int32 sums), reduce-then-scan.
-// Loads are coalesced: a wavefront owns 16 rows of 64 consecutive elements. Values come from a
-// source functor and results go to a sink functor, so producers and consumers fuse into the scan.
-
-constexpr int SCAN_BLOCK = 256;
-constexpr int SCAN_ROWS = 8;
-constexpr int SCAN_WAVE_ITEMS = WAVE * SCAN_ROWS;
-constexpr int SCAN_TILE = (SCAN_BLOCK / WAVE) * SCAN_WAVE_ITEMS;
-
-struct I2 {
-  int32_t x, y;
-};
-
-__device__ __forceinline__ I2 add2(I2 a, I2 b) { return I2{a.x + b.x, a.y + b.y}; }
-
-__device__ __forceinline__ I2 wave_sum2(I2 v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    v.x += __shfl_xor(v.x, d, WAVE);
-    v.y += __shfl_xor(v.y, d, WAVE);
-  }
-  return v;
-}
-
-__device__ __forceinline__ I2 wave_incl2(I2 v, int lane) {
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) {
-    const int32_t ox = __shfl_up(v.x, d, WAVE);
-    const int32_t oy = __shfl_up(v.y, d, WAVE);
-    if (lane >= d) {
-      v.x += ox;
-      v.y += oy;
-    }
-  }
-  return v;
-}
-
-struct SrcArray {
-  const I2 *a;
-  __device__ I2 operator()(uint64_t i) const { return a[i]; }
-};
-
-// +1 for a left (pushed) bracket, -1 for a right (popped) one (depths, parser.fut:153-166)
-struct SrcBracket {
-  const uint16_t *br;
-  __device__ I2 operator()(uint64_t i) const { return I2{(br[i] & 0x8000u) ? 1 : -1, 0}; }
-};
-
-// (arity - 1, is terminal production) (parents, parser.fut:286-296; terminal_offsets :305-314)
-struct SrcProd {
-  const uint16_t *prod;
-  const int32_t *arity;
-  const int32_t *p2t;
-  __device__ I2 operator()(uint64_t i) const {
-    const uint16_t p = prod[i];
-    return I2{arity[p] - 1, p2t[p] >= 0 ? 1 : 0};
-  }
-};
-
-struct SinkArray {  // exclusive prefix to an array
-  I2 *out;
-  __device__ void operator()(uint64_t i, bool valid, I2 v, I2 ex, I2 in, int lane) const {
-    if (valid) out[i] = ex;
-  }
-};
-
-// bracket depth = inclusive scan - [left]; reject on a negative prefix or a non-zero total;
-// 64-element minima of the depths (level 1 of the min hierarchy).
-struct SinkDepth {
-  int32_t *depth;
-  int32_t *l1;
-  uint32_t *flags;
-  uint64_t n;
-  __device__ void operator()(uint64_t i, bool valid, I2 v, I2 ex, I2 in, int lane) const {
-    int32_t d = INT32_MAX;
-    if (valid) {
-      if (in.x < 0 || (i == n - 1 && in.x != 0)) atomicOr(flags, 1u);
-      d = in.x - (v.x > 0 ? 1 : 0);
-      depth[i] = d;
-    }
-    const int32_t m = wave_min_i32(d);
-    if (lane == 0 && i < n) l1[i / WAVE] = m;
-  }
-};
-
-// s = exscan(arity - 1), terminal rank = exscan(is terminal); minima of s.
-struct SinkProd {
-  int32_t *s;
-  int32_t *rank;
-  int32_t *l1;
-  uint64_t n;
-  __device__ void operator()(uint64_t i, bool valid, I2 v, I2 ex, I2 in, int lane) const {
-    int32_t x = INT32_MAX;
-    if (valid) {
-      x = ex.x;
-      s[i] = x;
-      rank[i] = ex.y;
-    }
-    const int32_t m = wave_min_i32(x);
-    if (lane == 0 && i < n) l1[i / WAVE] = m;
-  }
-};
-
-template <class Src>
-__global__ void __launch_bounds__(SCAN_BLOCK) scan_reduce(Src src, uint64_t n, I2 *bsum) {
-  __shared__ I2 wsum[SCAN_BLOCK / WAVE];
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
-  I2 s{0, 0};
-#pragma unroll 4
-  for (int r = 0; r < SCAN_TILE / SCAN_BLOCK; ++r) {
-    const uint64_t i = base + (uint64_t)r * SCAN_BLOCK + threadIdx.x;
-    if (i < n) s = add2(s, src(i));
-  }
-  s = wave_sum2(s);
-  if ((threadIdx.x & (WAVE - 1)) == 0) wsum[threadIdx.x / WAVE] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    I2 t{0, 0};
-    for (int w = 0; w < SCAN_BLOCK / WAVE; ++w) t = add2(t, wsum[w]);
-    bsum[blockIdx.x] = t;
-  }
-}
-
-// one block of 1024 threads: exclusive scan of the block sums in place (each thread owns a
-// contiguous run); the grand total goes to bsum[nb]
-constexpr int SCANB_THREADS = 1024;
-__global__ void __launch_bounds__(SCANB_THREADS) scan_blocks(I2 *bsum, uint64_t nb) {
-  __shared__ I2 wsum[SCANB_THREADS / WAVE];
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int w = threadIdx.x / WAVE;
-  const uint64_t per = (nb + SCANB_THREADS - 1) / SCANB_THREADS;
-  const uint64_t lo = min(nb, (uint64_t)threadIdx.x * per), hi = min(nb, lo + per);
-  I2 t{0, 0};
-  for (uint64_t i = lo; i < hi; ++i) t = add2(t, bsum[i]);
-  const I2 incl = wave_incl2(t, lane);
-  if (lane == WAVE - 1) wsum[w] = incl;
-  __syncthreads();
-  I2 run{0, 0}, tot{0, 0};
-  for (int k = 0; k < SCANB_THREADS / WAVE; ++k) {
-    if (k < w) run = add2(run, wsum[k]);
-    tot = add2(tot, wsum[k]);
-  }
-  run = add2(run, incl);
-  run.x -= t.x;
-  run.y -= t.y;
-  for (uint64_t i = lo; i < hi; ++i) {
-    const I2 v = bsum[i];
-    bsum[i] = run;
-    run = add2(run, v);
-  }
-  if (threadIdx.x == 0) bsum[nb] = tot;
-}
-
-template <class Src, class Sink>
-__global__ void __launch_bounds__(SCAN_BLOCK) scan_down(Src src, uint64_t n, const I2 *bsum,
-                                                        Sink sink) {
-  __shared__ I2 wsum[SCAN_BLOCK / WAVE];
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int w = threadIdx.x / WAVE;
-  const uint64_t wbase = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)w * SCAN_WAVE_ITEMS;
-  I2 vals[SCAN_ROWS];
-  I2 tot{0, 0};
-#pragma unroll
-  for (int r = 0; r < SCAN_ROWS; ++r) {
-    const uint64_t i = wbase + (uint64_t)r * WAVE + lane;
-    vals[r] = i < n ? src(i) : I2{0, 0};
-    tot = add2(tot, vals[r]);
-  }
-  tot = wave_sum2(tot);
-  if (lane == 0) wsum[w] = tot;
-  __syncthreads();
-  I2 carry = bsum[blockIdx.x];
-  for (int k = 0; k < w; ++k) carry = add2(carry, wsum[k]);
-#pragma unroll
-  for (int r = 0; r < SCAN_ROWS; ++r) {
-    const uint64_t i = wbase + (uint64_t)r * WAVE + lane;
-    const I2 incl_row = wave_incl2(vals[r], lane);
-    const I2 in = add2(carry, incl_row);
-    I2 ex = in;
-    ex.x -= vals[r].x;
-    ex.y -= vals[r].y;
-    sink(i, i < n, vals[r], ex, in, lane);
-    carry.x += __shfl(incl_row.x, WAVE - 1, WAVE);
-    carry.y += __shfl(incl_row.y, WAVE - 1, WAVE);
-  }
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -975,67 +938,685 @@ __device__ int64_t prev_le_wave(const Levels &lv, uint64_t chunk, int lane, int3
 
 // ----------------------------------------------------------------------------------------------
 // parser kernels
+//
+// The LLP parser of futhark/parser.fut:235-336 over the pair table of the grammar. The
+// reference builds the bracket stream and the production stream by segmented copies of the pair
+// entries (segmented_copy, :216-233), then scans them element by element: bracket depths
+// (depths, :153-166), s = exscan(arity - 1) for the parents (:286-296) and the rank of every
+// terminal production (terminal_offsets, :305-314). Here every pair entry carries its totals
+// (brackets, productions, depth change, lowest depth prefix, sum of arity - 1, terminal
+// productions) and every bracket / production of an entry its value relative to the entry start,
+// so one scan over the T + 2 positions gives every element its depth, s and rank:
+//   parse_keys2   : key -> entry of every position, per-tile totals          (reads the tokens)
+//   parse_pscan   : exclusive scan of the tile totals, depth validity        (one block)
+//   parse_expand2 : per tile, the positions' prefixes, then every bracket / production element
+//                   written directly (depth + symbol; s, node and span), with 64-element minima
+//   bracket_match : brackets_matches (:174-184) by previous smaller-or-equal depth
+//   parse_parents : parents (:286-296) by previous smaller-or-equal s
 
-struct ParseArgs {
-  const uint32_t *kinds;
-  uint64_t T;  // tokens
-  uint32_t q, k, nT, start_t, end_t, empty_t;
-  const uint32_t *pair_index;
-  const uint32_t *pair_entry;
-  const uint16_t *stacks;
-  const uint16_t *prods;
-  const int32_t *arity;
-  const int32_t *p2t;
-  uint32_t *flags;  // [0] reject
+constexpr int PK_THREADS = 256;
+constexpr int PK_IPT = 4;
+constexpr int PK_TILE = PK_THREADS * PK_IPT;  // positions per tile
+constexpr int PK_MAXW = 16;                   // q + k at most
+constexpr int PK_IDX8_MAX = 52 * 1024;        // dense u8 pair index kept in LDS up to this size
+constexpr int PS_THREADS = 1024;              // parse_pscan block
+constexpr uint16_t PK_NO_ENTRY = 0xFFFF;      // position whose key has no pair entry
+
+// One LLP pair entry (16 bytes); mind = lowest inclusive depth prefix relative to the entry
+// start, INT16_MAX without brackets.
+struct EntInfo {
+  uint16_t nb, np;
+  int16_t dd, mind;
+  int16_t ds;
+  uint16_t nt;
+  uint16_t sst, pst;  // first bracket in stackw / first production in prodw
 };
 
-// keys over [start] ++ kinds ++ [end] (get_key / lookup, parser.fut:117-151)
-__global__ void parse_keys(ParseArgs a, uint32_t *ent, I2 *lens) {
-  const uint64_t P = a.T + 2;
-  for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < P;
-       p += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t di = 0;
-    for (uint32_t j = 0; j < a.q + a.k; ++j) {
-      const int64_t pos = (int64_t)p + (int64_t)j - (int64_t)a.q;
-      uint32_t t;
-      if (pos < 0 || pos >= (int64_t)P) t = a.empty_t;
-      else if (pos == 0) t = a.start_t;
-      else if (pos == (int64_t)P - 1) t = a.end_t;
-      else t = a.kinds[pos - 1];
-      di = di * a.nT + (t < a.nT ? t : a.empty_t);
-    }
-    const uint32_t id = a.pair_index[di];
-    if (id == 0xFFFFFFFFu) {
-      atomicOr(a.flags, 1u);
-      ent[p] = 0;
-      lens[p] = I2{0, 0};
+// Totals of a range of positions. All fields add, except mind, which is the lowest depth prefix
+// inside the range relative to its start (INT32_MAX when the range has no bracket).
+struct PTup {
+  int32_t nb, np, dd, mind, ds, nt;
+};
+
+__device__ __forceinline__ PTup ptup_id() { return PTup{0, 0, 0, INT32_MAX, 0, 0}; }
+
+__device__ __forceinline__ PTup ptup_op(const PTup &a, const PTup &b) {
+  PTup r;
+  r.nb = a.nb + b.nb;
+  r.np = a.np + b.np;
+  r.dd = a.dd + b.dd;
+  r.mind = b.mind == INT32_MAX ? a.mind : min(a.mind, a.dd + b.mind);
+  r.ds = a.ds + b.ds;
+  r.nt = a.nt + b.nt;
+  return r;
+}
+
+__device__ __forceinline__ PTup ptup_of(const EntInfo &e) {
+  return PTup{e.nb, e.np, e.dd, e.nb ? (int32_t)e.mind : INT32_MAX, e.ds, e.nt};
+}
+
+__device__ __forceinline__ PTup ptup_shfl_up(const PTup &v, int d) {
+  return PTup{__shfl_up(v.nb, d, WAVE), __shfl_up(v.np, d, WAVE), __shfl_up(v.dd, d, WAVE),
+              __shfl_up(v.mind, d, WAVE), __shfl_up(v.ds, d, WAVE), __shfl_up(v.nt, d, WAVE)};
+}
+
+__device__ __forceinline__ PTup ptup_shfl(const PTup &v, int src) {
+  return PTup{__shfl(v.nb, src, WAVE), __shfl(v.np, src, WAVE), __shfl(v.dd, src, WAVE),
+              __shfl(v.mind, src, WAVE), __shfl(v.ds, src, WAVE), __shfl(v.nt, src, WAVE)};
+}
+
+// inclusive scan across the wavefront, in lane order
+__device__ __forceinline__ PTup ptup_wave_incl(PTup v, int lane) {
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const PTup o = ptup_shfl_up(v, d);
+    if (lane >= d) v = ptup_op(o, v);
+  }
+  return v;
+}
+
+// Block-wide exclusive scan in thread order; returns the exclusive prefix of this thread and the
+// block total. s_w: LDS of at least blockDim.x / WAVE tuples.
+__device__ __forceinline__ PTup ptup_block_excl(PTup v, PTup *s_w, PTup &total) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = threadIdx.x / WAVE;
+  const int nw = blockDim.x / WAVE;
+  const PTup incl = ptup_wave_incl(v, lane);
+  if (lane == WAVE - 1) s_w[w] = incl;
+  __syncthreads();
+  PTup pre = ptup_id();
+  for (int i = 0; i < w; ++i) pre = ptup_op(pre, s_w[i]);
+  total = pre;
+  for (int i = w; i < nw; ++i) total = ptup_op(total, s_w[i]);
+  PTup ex = ptup_shfl_up(incl, 1);
+  if (lane == 0) ex = ptup_id();
+  __syncthreads();
+  return ptup_op(pre, ex);
+}
+
+struct PArgs {
+  const uint32_t *kinds;
+  uint64_t T;  // tokens
+  uint64_t P;  // positions: T + 2 (start and end terminal added, to_keys parser.fut:235-240)
+  uint32_t q, k, nT, start_t, end_t, empty_t;
+  const uint8_t *index8;    // dense pair index, entry or 0xFF (when E < 255)
+  const uint32_t *index32;  // dense pair index, entry or 0xFFFFFFFF
+  uint32_t index_len;
+  const EntInfo *einfo;
+  uint32_t E;
+  const uint32_t *stackw;  // bracket symbol (| 0x8000 left) | depth value relative to the entry << 16
+  const uint2 *prodw;      // x: production | (terminal + 1) << 16; y: s relative | rank relative << 16
+  uint32_t nstack, nprod;  // lengths of stackw / prodw
+  uint16_t *ent;           // [ntiles * PK_TILE] pair entry of every position
+  PTup *tiles;             // [ntiles] tile totals
+  PTup *pre;               // [ntiles] exclusive prefixes (parse_pscan)
+  PTup *total;             // [1]
+  uint32_t *flags;         // [0] reject
+  uint64_t ntiles;
+  int subst;               // single document: terminal productions pair up with the tokens
+  // batched mode: every document's positions start a new tile; element offsets run through the
+  // documents, depths, s and ranks restart in every document
+  struct PDoc *pdocs;        // [ndocs]
+  const uint32_t *ptile_doc; // [tiles] document of every position tile
+  const uint64_t *ntiles_dev;
+  uint32_t ndocs;
+  uint32_t *doc_flags;       // [ndocs] bit 0: missing pair, bit 1: bracket mismatch
+  uint32_t *chunk_doc;       // [N / 64] document of the first production of every 64-chunk
+};
+
+struct PDoc {
+  uint64_t tok_base;    // first token slot of the document (batched lexer layout)
+  uint64_t br_off;      // first bracket / production element (global)
+  uint64_t node_off;
+  uint64_t B, N;        // brackets, productions
+  uint32_t T;           // tokens
+  uint32_t lex_ok;
+  uint32_t ptile_base;  // first position tile
+  uint32_t ptiles;
+  uint32_t status;      // 1 while accepted so far
+  uint32_t subst;
+};
+
+// The positions a tile covers: the document's tokens, its position count, the tile's first
+// position inside the document.
+struct PosView {
+  const uint32_t *kinds;
+  uint64_t P;
+  int64_t p0;
+  uint64_t tok_base;
+  uint32_t doc;
+  bool reset;
+  int subst;
+};
+
+template <bool BATCH>
+__device__ __forceinline__ PosView pos_view(const PArgs &a, uint64_t tile) {
+  PosView v;
+  if (!BATCH) {
+    v.kinds = a.kinds;
+    v.P = a.P;
+    v.p0 = (int64_t)(tile * PK_TILE);
+    v.tok_base = 0;
+    v.doc = 0;
+    v.reset = tile == 0;
+    v.subst = a.subst;
+  } else {
+    const uint32_t d = a.ptile_doc[tile] & 0x7FFFFFFFu;
+    const PDoc &doc = a.pdocs[d];
+    const uint32_t lt = (uint32_t)tile - doc.ptile_base;
+    v.kinds = a.kinds + doc.tok_base;
+    v.P = (uint64_t)doc.T + 2;
+    v.p0 = (int64_t)lt * PK_TILE;
+    v.tok_base = doc.tok_base;
+    v.doc = d;
+    v.reset = lt == 0;
+    v.subst = (int)doc.subst;
+  }
+  return v;
+}
+
+// symbol of position x of [start] ++ kinds ++ [end], the empty terminal outside (get_key,
+// parser.fut:117-132); a kind the table does not know rejects
+__device__ __forceinline__ uint32_t pk_symbol(const PArgs &a, const uint32_t *kinds, uint64_t P,
+                                              int64_t x) {
+  if (x < 0 || x >= (int64_t)P) return a.empty_t;
+  if (x == 0) return a.start_t;
+  if (x == (int64_t)P - 1) return a.end_t;
+  return kinds[x - 1];
+}
+
+// keys and pair lookups (keys / lookup / to_keys, parser.fut:110-151, 235-240); per-tile totals.
+// Persistent, PKK_TILES tiles per block and iteration, one wavefront group of PK_THREADS threads per
+// tile; the u8 pair index is loaded into LDS once per block.
+constexpr int PKK_THREADS = 1024;
+constexpr int PKK_TILES = PKK_THREADS / PK_THREADS;
+
+template <bool IDX8, bool BATCH>
+__global__ void __launch_bounds__(PKK_THREADS) parse_keys2(PArgs a) {
+  extern __shared__ __align__(16) uint8_t pk_smem[];
+  __shared__ PTup s_w[PKK_THREADS / WAVE];
+  __shared__ EntInfo s_ei[256];
+  if (IDX8) {
+    for (uint32_t i = threadIdx.x; i < a.index_len; i += blockDim.x) pk_smem[i] = a.index8[i];
+    for (uint32_t i = threadIdx.x; i < a.E; i += blockDim.x) s_ei[i] = a.einfo[i];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = threadIdx.x / WAVE;
+  const uint32_t W = a.q + a.k;
+  const uint64_t ntiles = BATCH ? *a.ntiles_dev : a.ntiles;
+  const uint64_t nsuper = (ntiles + PKK_TILES - 1) / PKK_TILES;
+  bool miss = false;
+  for (uint64_t sup = blockIdx.x; sup < nsuper; sup += gridDim.x) {
+    const uint64_t my_tile = sup * PKK_TILES + threadIdx.x / PK_THREADS;
+    PosView pv;
+    if (my_tile < ntiles) {
+      pv = pos_view<BATCH>(a, my_tile);
     } else {
-      ent[p] = id;
-      const uint32_t *e4 = a.pair_entry + 4ull * id;
-      lens[p] = I2{(int32_t)(e4[1] - e4[0]), (int32_t)(e4[3] - e4[2])};
+      pv.kinds = a.kinds;
+      pv.P = 0;
+      pv.p0 = 0;
+      pv.doc = 0;
+    }
+    const int64_t lp0 = pv.p0 + (int64_t)(threadIdx.x % PK_THREADS) * PK_IPT;  // in the document
+    PTup acc = ptup_id();
+    uint16_t ids[PK_IPT];
+    // the window of symbols of this thread's positions
+    uint32_t sym[PK_IPT + PK_MAXW];
+#pragma unroll
+    for (int j = 0; j < PK_IPT + PK_MAXW - 1; ++j)
+      if (j < PK_IPT + (int)W - 1) sym[j] = pk_symbol(a, pv.kinds, pv.P, lp0 - (int64_t)a.q + j);
+    bool wmiss = false;
+#pragma unroll
+    for (int i = 0; i < PK_IPT; ++i) {
+      const int64_t p = lp0 + i;
+      ids[i] = 0;
+      if (p >= (int64_t)pv.P) continue;
+      uint64_t di = 0;
+      bool unknown = false;
+#pragma unroll
+      for (int j = 0; j < PK_MAXW; ++j) {
+        if (j < (int)W) {
+          const uint32_t t = sym[i + j];
+          unknown |= t >= a.nT;
+          di = di * a.nT + (t < a.nT ? t : 0u);
+        }
+      }
+      uint32_t id;
+      if (IDX8) {
+        id = unknown ? 0xFFu : pk_smem[di];
+        if (id == 0xFFu) id = 0xFFFFFFFFu;
+      } else {
+        id = unknown ? 0xFFFFFFFFu : a.index32[di];
+      }
+      if (id == 0xFFFFFFFFu) {
+        wmiss = true;
+        ids[i] = PK_NO_ENTRY;  // expands to nothing: the document is rejected anyway
+        continue;
+      }
+      ids[i] = (uint16_t)id;
+      acc = ptup_op(acc, ptup_of(IDX8 ? s_ei[id] : a.einfo[id]));
+    }
+    if (BATCH) {
+      if (__ballot(wmiss) != 0ull && lane == 0) atomicOr(a.doc_flags + pv.doc, 1u);
+    } else {
+      miss |= wmiss;
+    }
+    // slots are tile-major: tile * PK_TILE + position in the tile
+    const uint64_t pbase = sup * PKK_TILES * PK_TILE + (uint64_t)threadIdx.x * PK_IPT;
+    const uint64_t pend = my_tile * PK_TILE + (pv.P > (uint64_t)pv.p0 ? pv.P - (uint64_t)pv.p0 : 0);
+    if (pbase + PK_IPT <= pend) {
+      uint2 packed;
+      packed.x = (uint32_t)ids[0] | ((uint32_t)ids[1] << 16);
+      packed.y = (uint32_t)ids[2] | ((uint32_t)ids[3] << 16);
+      *reinterpret_cast<uint2 *>(a.ent + pbase) = packed;
+    } else {
+      for (int i = 0; i < PK_IPT; ++i)
+        if (pbase + i < pend) a.ent[pbase + i] = ids[i];
+    }
+    const PTup incl = ptup_wave_incl(acc, lane);
+    if (lane == WAVE - 1) s_w[w] = incl;
+    __syncthreads();
+    if (threadIdx.x < PKK_TILES) {
+      const uint64_t tile = sup * PKK_TILES + threadIdx.x;
+      if (tile < ntiles) {
+        const int wpt = PK_THREADS / WAVE;
+        PTup t = s_w[threadIdx.x * wpt];
+        for (int i = 1; i < wpt; ++i) t = ptup_op(t, s_w[threadIdx.x * wpt + i]);
+        a.tiles[tile] = t;
+      }
+    }
+    __syncthreads();
+  }
+  if (!BATCH && __ballot(miss) != 0ull && lane == 0) atomicOr(a.flags, 1u);
+}
+
+// Segmented tile totals: element counts (nb, np) run through all tiles, the other fields restart
+// at a tile that starts a document (batched mode).
+struct SegTup {
+  PTup t;
+  int f;
+};
+
+__device__ __forceinline__ SegTup seg_op(const SegTup &a, const SegTup &b) {
+  SegTup r;
+  if (b.f) {
+    r.t = b.t;
+    r.t.nb += a.t.nb;
+    r.t.np += a.t.np;
+    r.f = 1;
+  } else {
+    r.t = ptup_op(a.t, b.t);
+    r.f = a.f;
+  }
+  return r;
+}
+
+__device__ __forceinline__ SegTup seg_shfl_up(const SegTup &v, int d) {
+  return SegTup{ptup_shfl_up(v.t, d), __shfl_up(v.f, d, WAVE)};
+}
+
+// Tile scan into `pre` (exclusive) and the grand total, in three kernels: block totals of 1024
+// tiles, one block over the block totals, then every block again with its prefix. In a tile that
+// starts a document the prefix keeps only the element counts. The grand total decides the depth
+// check of depths (parser.fut:153-166) of a single document: no prefix below zero, and zero at
+// the end.
+constexpr int PSB = 1024;  // tiles per scan block
+
+template <bool BATCH>
+__device__ __forceinline__ int tile_reset(const PArgs &a, uint64_t t) {
+  if (!BATCH) return 0;
+  return (a.ptile_doc[t] >> 31) != 0 ? 1 : 0;
+}
+
+// block-wide segmented scan in thread order: exclusive prefix of this thread, block total
+__device__ __forceinline__ SegTup seg_block_excl(SegTup v, SegTup *s_w, SegTup &total) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = threadIdx.x / WAVE;
+  const int nw = blockDim.x / WAVE;
+  SegTup incl = v;
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const SegTup o = seg_shfl_up(incl, d);
+    if (lane >= d) incl = seg_op(o, incl);
+  }
+  if (lane == WAVE - 1) s_w[w] = incl;
+  __syncthreads();
+  SegTup pre{ptup_id(), 0};
+  for (int i = 0; i < w; ++i) pre = seg_op(pre, s_w[i]);
+  total = pre;
+  for (int i = w; i < nw; ++i) total = seg_op(total, s_w[i]);
+  SegTup ex = seg_shfl_up(incl, 1);
+  if (lane == 0) ex = SegTup{ptup_id(), 0};
+  __syncthreads();
+  return seg_op(pre, ex);
+}
+
+template <bool BATCH>
+__global__ void __launch_bounds__(PSB) pscan_reduce(PArgs a, SegTup *btot) {
+  __shared__ SegTup s_w[PSB / WAVE];
+  const uint64_t ntiles = BATCH ? *a.ntiles_dev : a.ntiles;
+  const uint64_t t = (uint64_t)blockIdx.x * PSB + threadIdx.x;
+  if ((uint64_t)blockIdx.x * PSB >= ntiles) return;  // whole block beyond the tiles
+  const SegTup v = t < ntiles ? SegTup{a.tiles[t], tile_reset<BATCH>(a, t)} : SegTup{ptup_id(), 0};
+  SegTup total;
+  (void)seg_block_excl(v, s_w, total);
+  if (threadIdx.x == 0) btot[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(PSB) pscan_top(PArgs a, SegTup *btot, uint64_t nblocks_max) {
+  __shared__ SegTup s_w[PSB / WAVE];
+  __shared__ SegTup s_carry;
+  const uint64_t ntiles = a.ntiles_dev ? *a.ntiles_dev : a.ntiles;
+  const uint64_t nb = std::min<uint64_t>(nblocks_max, (ntiles + PSB - 1) / PSB);
+  if (threadIdx.x == 0) s_carry = SegTup{ptup_id(), 0};
+  __syncthreads();
+  for (uint64_t off = 0; off < nb; off += PSB) {
+    const uint64_t i = off + threadIdx.x;
+    const SegTup v = i < nb ? btot[i] : SegTup{ptup_id(), 0};
+    SegTup total;
+    const SegTup ex = seg_block_excl(v, s_w, total);
+    const SegTup carry = s_carry;
+    if (i < nb) btot[i] = seg_op(carry, ex);
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry = seg_op(carry, total);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *a.total = s_carry.t;
+}
+
+template <bool BATCH>
+__global__ void __launch_bounds__(PSB) pscan_down(PArgs a, const SegTup *bpre) {
+  __shared__ SegTup s_w[PSB / WAVE];
+  const uint64_t ntiles = BATCH ? *a.ntiles_dev : a.ntiles;
+  const uint64_t t = (uint64_t)blockIdx.x * PSB + threadIdx.x;
+  if ((uint64_t)blockIdx.x * PSB >= ntiles) return;
+  const SegTup v = t < ntiles ? SegTup{a.tiles[t], tile_reset<BATCH>(a, t)} : SegTup{ptup_id(), 0};
+  SegTup total;
+  const SegTup ex = seg_op(bpre[blockIdx.x], seg_block_excl(v, s_w, total));
+  if (t < ntiles) {
+    PTup out = ex.t;
+    if (v.f) out = PTup{ex.t.nb, ex.t.np, 0, INT32_MAX, 0, 0};
+    a.pre[t] = out;
+  }
+}
+
+// Batched parser set-up: per document, positions T + 2 when the lexer accepted it, in tiles of
+// PK_TILE; the first tile of every document (exclusive sum) and the tile count.
+__global__ void __launch_bounds__(PS_THREADS) pb_setup(PArgs a, const LexDoc *ldocs,
+                                                        const uint32_t *doc_res,
+                                                        uint64_t *ntiles_out) {
+  __shared__ uint32_t s_w[PS_THREADS / WAVE];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = threadIdx.x / WAVE;
+  const uint32_t D = a.ndocs;
+  const uint32_t run = (D + PS_THREADS - 1) / PS_THREADS;
+  const uint32_t lo = std::min<uint32_t>(D, threadIdx.x * run);
+  const uint32_t hi = std::min<uint32_t>(D, lo + run);
+  uint32_t acc = 0;
+  for (uint32_t d = lo; d < hi; ++d) {
+    const uint32_t ok = doc_res[2 * d];
+    const uint32_t T = doc_res[2 * d + 1];
+    acc += ok ? (uint32_t)((T + 2 + PK_TILE - 1) / PK_TILE) : 0u;
+  }
+  uint32_t incl = acc;
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d, WAVE);
+    if (lane >= d) incl += o;
+  }
+  if (lane == WAVE - 1) s_w[w] = incl;
+  __syncthreads();
+  uint32_t base = incl - acc;
+  for (int i = 0; i < w; ++i) base += s_w[i];
+  for (uint32_t d = lo; d < hi; ++d) {
+    const uint32_t ok = doc_res[2 * d];
+    const uint32_t T = doc_res[2 * d + 1];
+    PDoc pd;
+    pd.tok_base = ldocs[d].tok_base;
+    pd.br_off = pd.node_off = pd.B = pd.N = 0;
+    pd.T = T;
+    pd.lex_ok = ok;
+    pd.ptile_base = base;
+    pd.ptiles = ok ? (uint32_t)((T + 2 + PK_TILE - 1) / PK_TILE) : 0u;
+    pd.status = ok;
+    pd.subst = 0;
+    a.pdocs[d] = pd;
+    a.doc_flags[d] = 0;
+    base += pd.ptiles;
+  }
+  if (threadIdx.x == PS_THREADS - 1) *ntiles_out = base;
+}
+
+// Batched parser: the document of every position tile.
+__global__ void pb_tile_docs(PArgs a) {
+  const PDoc &doc = a.pdocs[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < doc.ptiles; i += blockDim.x)
+    const_cast<uint32_t *>(a.ptile_doc)[doc.ptile_base + i] = blockIdx.x | (i == 0 ? 0x80000000u : 0u);
+}
+
+// Batched parser: per document element ranges, the depth check (depths, parser.fut:153-166) and
+// whether the terminal productions pair up with the tokens (safe_zip, :300-303).
+__global__ void pb_docs(PArgs a) {
+  const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= a.ndocs) return;
+  PDoc pd = a.pdocs[d];
+  if (pd.ptiles == 0) {
+    // no positions: the offsets where the document's elements would start keep them monotone
+    const uint64_t ntiles = *a.ntiles_dev;
+    const PTup at = pd.ptile_base < ntiles ? a.pre[pd.ptile_base] : *a.total;
+    pd.br_off = (uint32_t)at.nb;
+    pd.node_off = (uint32_t)at.np;
+    pd.status = 0;
+    a.pdocs[d] = pd;
+    return;
+  }
+  const uint32_t f = pd.ptile_base, l = f + pd.ptiles - 1;
+  const PTup first = a.pre[f];
+  PTup inc = ptup_op(a.pre[l], a.tiles[l]);
+  pd.br_off = (uint32_t)first.nb;
+  pd.node_off = (uint32_t)first.np;
+  pd.B = (uint32_t)(inc.nb - first.nb);
+  pd.N = (uint32_t)(inc.np - first.np);
+  bool ok = pd.lex_ok && (a.doc_flags[d] & 1u) == 0;
+  if (pd.B > 0 && (inc.mind < 0 || inc.dd != 0)) ok = false;
+  pd.status = ok ? 1u : 0u;
+  pd.subst = (uint32_t)inc.nt == pd.T ? 1u : 0u;
+  a.pdocs[d] = pd;
+}
+
+constexpr int PX_STAGE = 256;  // elements staged per wavefront and round
+
+// Per tile: the positions' prefixes (block scan), then the brackets and productions of every
+// wavefront's positions. A lane's 4 positions own one contiguous run of elements; each round
+// stages the elements of a 64-aligned window of PX_STAGE in LDS and copies them out coalesced,
+// with the minimum of every 64-element chunk (chunks shared with a neighbouring wavefront go
+// through atomicMin on an array cleared to INT32_MAX).
+template <typename S, bool BATCH>
+__global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *starts, const S *ends,
+                                                            int32_t *depth, uint16_t *br,
+                                                            int32_t *dl1, int32_t *sx, int32_t *sl1,
+                                                            uint32_t *node, S *span,
+                                                            int write_nodes) {
+  __shared__ PTup s_w[PK_THREADS / WAVE];
+  __shared__ uint32_t s_stage[PK_THREADS / WAVE][3][PX_STAGE];
+  extern __shared__ __align__(16) uint8_t px_smem[];  // stackw, then prodw
+  uint32_t *s_stackw = reinterpret_cast<uint32_t *>(px_smem);
+  uint2 *s_prodw = reinterpret_cast<uint2 *>(px_smem + 4 * ((a.nstack + 1) & ~1u));
+  for (uint32_t i = threadIdx.x; i < a.nstack; i += blockDim.x) s_stackw[i] = a.stackw[i];
+  for (uint32_t i = threadIdx.x; i < a.nprod; i += blockDim.x) s_prodw[i] = a.prodw[i];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = threadIdx.x / WAVE;
+  const uint64_t tile = blockIdx.x;
+  if (BATCH && tile >= *a.ntiles_dev) return;  // the grid covers the largest possible tile count
+  __syncthreads();  // tables in LDS
+  const PosView pv = pos_view<BATCH>(a, tile);
+  const int subst = pv.subst;
+  const PTup pre = a.pre[tile];
+  const uint64_t pbase = tile * PK_TILE + (uint64_t)threadIdx.x * PK_IPT;  // slot
+  const uint64_t pend = tile * PK_TILE + (pv.P > (uint64_t)pv.p0 ? pv.P - (uint64_t)pv.p0 : 0);
+  uint16_t ids[PK_IPT] = {0, 0, 0, 0};
+  if (pbase + PK_IPT <= pend) {
+    const uint2 packed = *reinterpret_cast<const uint2 *>(a.ent + pbase);
+    ids[0] = (uint16_t)packed.x;
+    ids[1] = (uint16_t)(packed.x >> 16);
+    ids[2] = (uint16_t)packed.y;
+    ids[3] = (uint16_t)(packed.y >> 16);
+  } else {
+    for (int i = 0; i < PK_IPT; ++i)
+      if (pbase + i < pend) ids[i] = a.ent[pbase + i];
+  }
+  EntInfo inf[PK_IPT];
+  PTup acc = ptup_id();
+#pragma unroll
+  for (int i = 0; i < PK_IPT; ++i) {
+    inf[i] = (pbase + i < pend && ids[i] != PK_NO_ENTRY) ? a.einfo[ids[i]]
+                                                         : EntInfo{0, 0, 0, INT16_MAX, 0, 0, 0, 0};
+    acc = ptup_op(acc, ptup_of(inf[i]));
+  }
+  PTup total;
+  const PTup ex = ptup_block_excl(acc, s_w, total);  // also orders the table loads above
+  // element prefixes of this lane's positions (global)
+  int32_t ob[PK_IPT], op[PK_IPT], d0[PK_IPT], s0[PK_IPT], r0[PK_IPT];
+  {
+    int32_t b = pre.nb + ex.nb, q = pre.np + ex.np, d = pre.dd + ex.dd, sv = pre.ds + ex.ds,
+            r = pre.nt + ex.nt;
+#pragma unroll
+    for (int i = 0; i < PK_IPT; ++i) {
+      ob[i] = b;
+      op[i] = q;
+      d0[i] = d;
+      s0[i] = sv;
+      r0[i] = r;
+      b += inf[i].nb;
+      q += inf[i].np;
+      d += inf[i].dd;
+      sv += inf[i].ds;
+      r += inf[i].nt;
+    }
+  }
+  const int32_t lb_hi = ob[PK_IPT - 1] + inf[PK_IPT - 1].nb;  // this lane's brackets: [ob[0], lb_hi)
+  const int32_t lp_hi = op[PK_IPT - 1] + inf[PK_IPT - 1].np;
+  uint32_t *st0 = s_stage[w][0];
+  uint32_t *st1 = s_stage[w][1];
+  uint32_t *st2 = s_stage[w][2];
+
+  // brackets: depth value (depths, parser.fut:153-166) and symbol
+  {
+    const int32_t wlo = __shfl(ob[0], 0, WAVE), whi = __shfl(lb_hi, WAVE - 1, WAVE);
+    for (int32_t r0w = wlo & ~(WAVE - 1); r0w < whi; r0w += PX_STAGE) {
+      const int32_t glo = max(ob[0], r0w), ghi = min(lb_hi, r0w + PX_STAGE);
+      for (int32_t g = glo; g < ghi; ++g) {
+        const int i = g >= ob[3] ? 3 : g >= ob[2] ? 2 : g >= ob[1] ? 1 : 0;
+        const int32_t obi = i == 3 ? ob[3] : i == 2 ? ob[2] : i == 1 ? ob[1] : ob[0];
+        const int32_t di = i == 3 ? d0[3] : i == 2 ? d0[2] : i == 1 ? d0[1] : d0[0];
+        const uint32_t ssti = i == 3 ? inf[3].sst : i == 2 ? inf[2].sst : i == 1 ? inf[1].sst : inf[0].sst;
+        const uint32_t sw = s_stackw[ssti + (uint32_t)(g - obi)];
+        st0[g - r0w] = (uint32_t)(di + (int32_t)(int16_t)(sw >> 16));
+        st1[g - r0w] = sw & 0xFFFFu;
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int j = 0; j < PX_STAGE / WAVE; ++j) {
+        const int32_t cb = r0w + j * WAVE;
+        if (cb >= whi) break;
+        const int32_t g = cb + lane;
+        const bool act = g >= wlo && g < whi;
+        int32_t dv = INT32_MAX;
+        if (act) {
+          dv = (int32_t)st0[j * WAVE + lane];
+          depth[g] = dv;
+          br[g] = (uint16_t)st1[j * WAVE + lane];
+        }
+        const int32_t m = wave_min_i32(dv);
+        if (lane == 0) {
+          if (cb >= wlo && cb + WAVE <= whi) dl1[cb / WAVE] = m;
+          else atomicMin(dl1 + cb / WAVE, m);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+  // productions: s (parents, parser.fut:286-296), node and span (terminal_offsets, :305-332)
+  {
+    const int32_t wlo = __shfl(op[0], 0, WAVE), whi = __shfl(lp_hi, WAVE - 1, WAVE);
+    for (int32_t r0w = wlo & ~(WAVE - 1); r0w < whi; r0w += PX_STAGE) {
+      const int32_t glo = max(op[0], r0w), ghi = min(lp_hi, r0w + PX_STAGE);
+      for (int32_t g = glo; g < ghi; ++g) {
+        const int i = g >= op[3] ? 3 : g >= op[2] ? 2 : g >= op[1] ? 1 : 0;
+        const int32_t opi = i == 3 ? op[3] : i == 2 ? op[2] : i == 1 ? op[1] : op[0];
+        const int32_t si = i == 3 ? s0[3] : i == 2 ? s0[2] : i == 1 ? s0[1] : s0[0];
+        const int32_t ri = i == 3 ? r0[3] : i == 2 ? r0[2] : i == 1 ? r0[1] : r0[0];
+        const uint32_t psti = i == 3 ? inf[3].pst : i == 2 ? inf[2].pst : i == 1 ? inf[1].pst : inf[0].pst;
+        const uint2 pw = s_prodw[psti + (uint32_t)(g - opi)];
+        const int32_t t = subst ? (int32_t)(pw.x >> 16) - 1 : -1;
+        st0[g - r0w] = (uint32_t)(si + (int32_t)(int16_t)(pw.y & 0xFFFFu));
+        st1[g - r0w] = t >= 0 ? (ALPACC_NODE_TERMINAL | (uint32_t)t) : (pw.x & 0xFFFFu);
+        st2[g - r0w] = t >= 0 ? (uint32_t)pv.tok_base + (uint32_t)(ri + (int32_t)(pw.y >> 16))
+                              : 0xFFFFFFFFu;
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int j = 0; j < PX_STAGE / WAVE; ++j) {
+        const int32_t cb = r0w + j * WAVE;
+        if (cb >= whi) break;
+        const int32_t g = cb + lane;
+        const bool act = g >= wlo && g < whi;
+        int32_t sv = INT32_MAX;
+        if (act) {
+          sv = (int32_t)st0[j * WAVE + lane];
+          sx[g] = sv;
+          if (write_nodes) {
+            const uint32_t r = st2[j * WAVE + lane];
+            S v0 = 0, v1 = 0;
+            if (r != 0xFFFFFFFFu) {
+              v0 = starts[r];
+              v1 = ends[r];
+            }
+            node[g] = st1[j * WAVE + lane];
+            if (sizeof(S) == 4) {
+              reinterpret_cast<uint2 *>(span)[g] = make_uint2((uint32_t)v0, (uint32_t)v1);
+            } else {
+              span[2 * (uint64_t)g] = v0;
+              span[2 * (uint64_t)g + 1] = v1;
+            }
+          }
+        }
+        const int32_t m = wave_min_i32(sv);
+        if (lane == 0) {
+          if (cb >= wlo && cb + WAVE <= whi) sl1[cb / WAVE] = m;
+          else atomicMin(sl1 + cb / WAVE, m);
+          if (BATCH && cb >= wlo) a.chunk_doc[cb / WAVE] = pv.doc;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
 }
 
-// segmented copy of stack and production spans (segmented_copy, parser.fut:216-233)
-__global__ void parse_expand(ParseArgs a, const uint32_t *ent, const I2 *offs, uint16_t *br,
-                             uint16_t *prod) {
-  const uint64_t P = a.T + 2;
-  for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < P;
-       p += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t *e4 = a.pair_entry + 4ull * ent[p];
-    const I2 o = offs[p];
-    for (uint32_t s = e4[0]; s < e4[1]; ++s) br[(uint64_t)o.x + (s - e4[0])] = a.stacks[s];
-    for (uint32_t s = e4[2]; s < e4[3]; ++s) prod[(uint64_t)o.y + (s - e4[2])] = a.prods[s];
-  }
+__global__ void fill_i32(int32_t *p, uint64_t n, int32_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 // brackets_matches (parser.fut:174-184): every right bracket equals (without the left bit) the
-// bracket at its previous smaller-or-equal depth. One wavefront per 64 brackets.
-__global__ void bracket_match(const uint16_t *br, uint64_t B, Levels lv, uint32_t *flags) {
+// bracket at its previous smaller-or-equal depth. One wavefront per 64 brackets; a match inside
+// the wavefront's own chunk is read from a register.
+template <bool BATCH>
+__global__ void bracket_match(const uint16_t *br, uint64_t B, Levels lv, uint32_t *flags, PArgs a) {
   const int lane = threadIdx.x & (WAVE - 1);
   const uint64_t nch = (B + WAVE - 1) / WAVE;
   const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
+  bool bad = false;
   for (uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch;
        c += wstride) {
     const uint64_t i = c * WAVE + lane;
@@ -1045,18 +1626,37 @@ __global__ void bracket_match(const uint16_t *br, uint64_t B, Levels lv, uint32_
     const int32_t wv = window_min(lv, c, lane);
     const bool right = valid && !(b & 0x8000u);
     const int64_t j = prev_le_wave(lv, c, lane, v, wv, right);
-    if (right && (j < 0 || (br[j] & 0x7FFFu) != (b & 0x7FFFu))) atomicOr(flags, 1u);
+    const int64_t jl = j - (int64_t)(c * WAVE);
+    const uint16_t bin = (uint16_t)__shfl((int)b, (int)(jl & (WAVE - 1)), WAVE);
+    bool mism = false;
+    if (right) {
+      if (j < 0) {
+        mism = true;
+      } else {
+        const uint16_t bj = (jl >= 0 && jl < WAVE) ? bin : br[j];
+        mism = (bj & 0x7FFFu) != (b & 0x7FFFu);
+      }
+    }
+    if (BATCH && mism) {
+      // the document holding bracket i: the last d with br_off <= i (offsets are monotone)
+      uint32_t lo = 0, hi = a.ndocs;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (a.pdocs[mid].br_off <= i) lo = mid;
+        else hi = mid;
+      }
+      atomicOr(a.doc_flags + lo, 2u);
+    }
+    bad |= mism;
   }
+  if (!BATCH && __ballot(bad) != 0ull && lane == 0) atomicOr(flags, 1u);
 }
 
-// parents (parser.fut:286-296) and terminal nodes (parser.fut:305-332). One wavefront per chunk.
-// S: span word (u32, or u64 for the chunked lexer's spans).
-// subst = 0 when the terminal productions do not pair up with the tokens: safe_zip then yields no
-// pairs (parser.fut:300-303) and every node stays a production node.
-template <typename S>
-__global__ void parse_nodes(ParseArgs a, const uint16_t *prod, const int32_t *rank, uint64_t N,
-                            Levels lv, const S *starts, const S *ends, uint32_t *parent,
-                            uint32_t *node, S *span, int subst) {
+// parents (parser.fut:286-296): previous smaller-or-equal s, the root its own parent.
+// Batched: parents are relative to the document's first node, every document's first node is
+// its own parent.
+template <bool BATCH>
+__global__ void parse_parents(uint64_t N, Levels lv, uint32_t *parent, PArgs a) {
   const int lane = threadIdx.x & (WAVE - 1);
   const uint64_t nch = (N + WAVE - 1) / WAVE;
   const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
@@ -1064,27 +1664,17 @@ __global__ void parse_nodes(ParseArgs a, const uint16_t *prod, const int32_t *ra
        c += wstride) {
     const uint64_t i = c * WAVE + lane;
     const bool valid = i < N;
+    uint64_t base = 0;
+    if (BATCH) {
+      // the document of node i: from the document of the chunk's first node forward
+      uint32_t d = a.chunk_doc[c];
+      while (d + 1 < a.ndocs && a.pdocs[d + 1].node_off <= i && valid) ++d;
+      base = a.pdocs[d].node_off;
+    }
     const int32_t v = valid ? lv.L[0][i] : INT32_MAX;
     const int32_t wv = window_min(lv, c, lane);
-    // loads that do not depend on the search go out first
-    const uint16_t pr = valid ? prod[i] : (uint16_t)0;
-    const int32_t t = valid && subst ? a.p2t[pr] : -1;
-    S s0 = 0, s1 = 0;
-    if (t >= 0) {
-      const uint32_t r = (uint32_t)rank[i];
-      s0 = starts[r];
-      s1 = ends[r];
-    }
-    const int64_t j = prev_le_wave(lv, c, lane, v, wv, valid && i > 0);
-    if (!valid) continue;
-    parent[i] = i == 0 ? 0u : (uint32_t)j;
-    node[i] = t >= 0 ? (ALPACC_NODE_TERMINAL | (uint32_t)t) : (uint32_t)pr;
-    if (sizeof(S) == 4) {
-      reinterpret_cast<uint2 *>(span)[i] = make_uint2((uint32_t)s0, (uint32_t)s1);
-    } else {
-      span[2 * i] = s0;
-      span[2 * i + 1] = s1;
-    }
+    const int64_t j = prev_le_wave(lv, c, lane, v, wv, valid && i > base);
+    if (valid) parent[i] = i == base ? 0u : (uint32_t)((uint64_t)j - base);
   }
 }
 
@@ -1101,8 +1691,20 @@ inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 16) {
 
 static std::string g_create_error;
 
-enum Stage { S_LEX_REDUCE = 0, S_LEX_SCAN, S_LEX_EMIT, S_PARSE, NSTAGE };
-static const char *kStageNames[NSTAGE] = {"lex_reduce", "lex_scan", "lex_emit", "parse"};
+enum Stage {
+  S_LEX_REDUCE = 0,
+  S_LEX_SCAN,
+  S_LEX_EMIT,
+  S_PARSE,          // the whole parser stage
+  S_PARSE_KEYS,     // parse_keys2 + the tile scan
+  S_PARSE_EXPAND,   // parse_expand2
+  S_BRACKET_MATCH,  // bracket_match
+  S_PARSE_PARENTS,  // parse_parents
+  NSTAGE
+};
+static const char *kStageNames[NSTAGE] = {"lex_reduce",   "lex_scan",      "lex_emit",
+                                          "parse",        "parse_keys",    "parse_expand",
+                                          "bracket_match", "parse_parents"};
 
 struct alpacc_ctx {
   int device = 0;
@@ -1120,10 +1722,15 @@ struct alpacc_ctx {
   size_t lex_lds = 0;
   // parser tables
   bool has_parser = false;
-  uint32_t q = 0, k = 0, nT = 0, start_t = 0, end_t = 0, empty_t = 0, P = 0;
-  uint32_t *d_pair_index = nullptr, *d_pair_entry = nullptr;
-  uint16_t *d_stacks = nullptr, *d_prods = nullptr;
-  int32_t *d_arity = nullptr, *d_p2t = nullptr;
+  uint32_t q = 0, k = 0, nT = 0, start_t = 0, end_t = 0, empty_t = 0, E = 0;
+  uint32_t *d_pair_index = nullptr;  // dense index, u32 entries
+  uint8_t *d_index8 = nullptr;       // the same as u8 (E < 255 and small enough for LDS)
+  size_t index_len = 0;
+  EntInfo *d_einfo = nullptr;
+  uint32_t *d_stackw = nullptr;
+  uint2 *d_prodw = nullptr;
+  uint32_t max_nb = 0, max_np = 0;
+  size_t nstackw = 0, nprodw = 0;
   // scratch (grow-only)
   std::vector<std::pair<void *, size_t>> scratch;
   uint32_t *h_small = nullptr;  // pinned host words for results
@@ -1143,7 +1750,13 @@ struct alpacc_ctx {
       rec[s] = true;
     }
   }
-  void prof_drop(int s) { rec[s] = false; }
+  void prof_drop(int s) {
+    rec[s] = false;
+    if (s == S_PARSE)
+      for (int i = S_PARSE_KEYS; i <= S_PARSE_PARENTS; ++i) rec[i] = false;
+  }
+  // element counts of the last call: tokens, positions, brackets, productions
+  uint64_t last_counts[4] = {0, 0, 0, 0};
   void prof_collect() {
     for (int s = 0; s < NSTAGE; ++s) {
       float ms = 0.f;
@@ -1184,14 +1797,10 @@ enum BufId {
   B_TOK_START,
   B_TOK_END,
   B_ENT,
-  B_LENS,
-  B_OFFS,
-  B_BSUM,
+  B_PTILES,
   B_BR,
-  B_PROD,
   B_DEPTH,
   B_LEVELS,
-  B_RANK,
   B_SX,
   B_DL1,
   B_SL1,
@@ -1199,6 +1808,16 @@ enum BufId {
   B_CHUNK_INFO,
   B_TOK_START64,
   B_TOK_END64,
+  B_DOCS,
+  B_TILE_DOC,
+  B_DOC_RES,
+  B_PPRE,
+  B_BTOT,
+  B_PDOCS,
+  B_DOC_FLAGS,
+  B_PTILE_DOC,
+  B_RESULTS,
+  B_CHUNK_DOC,
 };
 
 template <typename T>
@@ -1227,34 +1846,6 @@ T *upload(const T *h, size_t count, std::string &err) {
 
 // reduce-then-scan of n int2 values from `src` into `sink`; the grand total goes to *total
 // (copied to the host when total is not NULL).
-template <class Src, class Sink>
-int scan2(alpacc_ctx *ctx, Src src, Sink sink, uint64_t n, I2 *total, hipStream_t st) {
-  const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-  I2 *bsum = (I2 *)ctx->buf(B_BSUM, (nb + 1) * sizeof(I2));
-  if (!bsum) {
-    ctx->err = "scratch allocation failed";
-    return ALPACC_ERR_HIP;
-  }
-  if (n > 0) {
-    scan_reduce<Src><<<(unsigned)nb, SCAN_BLOCK, 0, st>>>(src, n, bsum);
-    scan_blocks<<<1, SCANB_THREADS, 0, st>>>(bsum, nb);
-    scan_down<Src, Sink><<<(unsigned)nb, SCAN_BLOCK, 0, st>>>(src, n, bsum, sink);
-    ctx->launches += 3;
-    CK(hipGetLastError());
-  }
-  if (total) {
-    if (n == 0) {
-      *total = I2{0, 0};
-    } else {
-      CK(hipMemcpyAsync(total, bsum + nb, sizeof(I2), hipMemcpyDeviceToHost, st));
-      CK(hipStreamSynchronize(st));
-    }
-  }
-  return 0;
-}
-
-// Level arrays of the min hierarchy over `base` (n values): level 1 must already be in
-// `l1`; levels 2.. are built here. Every level is padded so that 16-byte quads can be read.
 int build_levels(alpacc_ctx *ctx, int32_t *base, int32_t *l1, uint64_t n, Levels &lv,
                  hipStream_t st) {
   lv.nlev = 1;
@@ -1389,14 +1980,18 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   a.starts64 = starts64;
   a.ends64 = ends64;
   a.chunk_info = chunk_info;
+  a.docs = nullptr;
+  a.tile_doc = nullptr;
+  a.ndocs = 0;
+  a.doc_res = nullptr;
   const uint64_t wpb = LEX_BLOCK / WAVE;
   uint64_t blocks = (tiles + wpb - 1) / wpb;
   blocks = std::min<uint64_t>(blocks, (uint64_t)std::max(1, ctx->cus) * 2);
   ctx->prof_begin(S_LEX_REDUCE, st);
   if (ctx->comp_in_lds)
-    lex_reduce<true><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
+    lex_reduce<true, false><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
   else
-    lex_reduce<false><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
+    lex_reduce<false, false><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
   ctx->prof_end(S_LEX_REDUCE, st);
   ctx->prof_begin(S_LEX_SCAN, st);
   lex_scan_tiles<<<(unsigned)sblocks, LEX_SCAN_BLOCK, 0, st>>>(a);
@@ -1405,11 +2000,11 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   ctx->prof_end(S_LEX_SCAN, st);
   ctx->prof_begin(S_LEX_EMIT, st);
   if (chunk > 0) {
-    lex_emit<true><<<(unsigned)ctiles, LEX_BLOCK, 0, st>>>(a);
+    lex_emit<true, false><<<(unsigned)ctiles, LEX_BLOCK, 0, st>>>(a);
     lex_chunk_fixup<<<grid_for(K, 256, 1024), 256, 0, st>>>(a);
     ctx->launches++;
   } else {
-    lex_emit<false><<<(unsigned)ctiles, LEX_BLOCK, 0, st>>>(a);
+    lex_emit<false, false><<<(unsigned)ctiles, LEX_BLOCK, 0, st>>>(a);
   }
   ctx->prof_end(S_LEX_EMIT, st);
   ctx->launches += 7;
@@ -1424,6 +2019,178 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   return ctx->h_small[0] ? ALPACC_ACCEPT : ALPACC_REJECT;
 }
 
+// Batched lexer: documents (offset, length) of d_in, each lexed on its own; document d's tokens go
+// to slots [tok_base_d, tok_base_d + len_d + 1) of kinds/starts/ends (tok_base = exclusive sum of
+// len + 1), its (accept, token count) to doc_res[2d], [2d+1] (device). `docs` is filled here.
+int run_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
+                  const uint64_t *lengths, uint32_t ndocs, uint32_t *kinds, uint32_t *starts,
+                  uint32_t *ends, uint32_t *doc_res, LexDoc **d_docs_out, hipStream_t st) {
+  if (!ctx->has_lexer) {
+    ctx->err = "bundle has no lexer";
+    return ALPACC_ERR_ARG;
+  }
+  std::vector<LexDoc> docs(ndocs);
+  uint64_t tiles = 0, toks = 0;
+  for (uint32_t d = 0; d < ndocs; ++d) {
+    if (lengths[d] >= (1ull << 31) - 2) {
+      ctx->err = "documents must be shorter than 2^31 - 2 bytes";
+      return ALPACC_ERR_ARG;
+    }
+    const uint64_t nt = (lengths[d] + LEX_TILE - 1) / LEX_TILE;
+    docs[d] = LexDoc{offsets[d], lengths[d], toks, (uint32_t)tiles, (uint32_t)nt};
+    tiles += nt;
+    toks += lengths[d] + 1;
+    if (tiles >= (1ull << 31) / LEX_TILE * 2) {
+      ctx->err = "batch too large";
+      return ALPACC_ERR_ARG;
+    }
+  }
+  LexDoc *d_docs = (LexDoc *)ctx->buf(B_DOCS, (ndocs + 1) * sizeof(LexDoc));
+  uint32_t *small = (uint32_t *)ctx->buf(B_SMALL, 64 * sizeof(uint32_t));
+  if (!d_docs || !small) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  CK(hipMemcpyAsync(d_docs, docs.data(), ndocs * sizeof(LexDoc), hipMemcpyHostToDevice, st));
+  *d_docs_out = d_docs;
+  clear_u32<<<grid_for(2 * (uint64_t)ndocs, 256), 256, 0, st>>>(doc_res, 2 * (uint64_t)ndocs);
+  clear_u32<<<1, 64, 0, st>>>(small, 16);
+  ctx->launches += 2;
+  if (tiles == 0) {
+    CK(hipGetLastError());
+    return 0;
+  }
+  const uint64_t ctiles = (tiles + LEX_CT_WAVES - 1) / LEX_CT_WAVES;
+  uint16_t *lane_excl = (uint16_t *)ctx->buf(B_LANE_EXCL, tiles * WAVE * sizeof(uint16_t));
+  uint16_t *tile_agg = (uint16_t *)ctx->buf(B_TILE_AGG, tiles * sizeof(uint16_t));
+  uint8_t *tile_state = (uint8_t *)ctx->buf(B_TILE_STATE, tiles);
+  uint64_t *st_count = (uint64_t *)ctx->buf(B_ST_COUNT, ctiles * sizeof(uint64_t));
+  uint32_t *tile_doc = (uint32_t *)ctx->buf(B_TILE_DOC, tiles * sizeof(uint32_t));
+  if (!lane_excl || !tile_agg || !tile_state || !st_count || !tile_doc) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  clear_u32<<<grid_for(2 * ctiles, 256), 256, 0, st>>>((uint32_t *)st_count, 2 * ctiles);
+  LexArgs a;
+  memset(&a, 0, sizeof(a));
+  a.in = d_in;
+  a.n = 0;
+  a.kinds = kinds;
+  a.starts = starts;
+  a.ends = ends;
+  a.byte_class = ctx->d_cls;
+  a.endo_step = ctx->d_step;
+  a.endo_comp = ctx->d_comp;
+  a.endo_apply = ctx->d_apply;
+  a.dfa_step = ctx->d_dfa;
+  a.state_accept = ctx->d_accept;
+  a.e = ctx->e;
+  a.C = ctx->C;
+  a.m1 = ctx->m1;
+  a.init = ctx->init;
+  a.ignore = ctx->ignore;
+  a.num_tiles = (uint32_t)tiles;
+  a.num_ctiles = (uint32_t)ctiles;
+  a.lane_excl = lane_excl;
+  a.tile_agg = tile_agg;
+  a.tile_state = tile_state;
+  a.tile_counter = small;
+  a.st_count = st_count;
+  a.result = small + 4;
+  a.docs = d_docs;
+  a.tile_doc = tile_doc;
+  a.ndocs = ndocs;
+  a.doc_res = doc_res;
+  lex_tile_docs<<<ndocs, 256, 0, st>>>(a);
+  const uint64_t wpb = LEX_BLOCK / WAVE;
+  uint64_t blocks = (tiles + wpb - 1) / wpb;
+  blocks = std::min<uint64_t>(blocks, (uint64_t)std::max(1, ctx->cus) * 2);
+  ctx->prof_begin(S_LEX_REDUCE, st);
+  if (ctx->comp_in_lds)
+    lex_reduce<true, true><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
+  else
+    lex_reduce<false, true><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
+  ctx->prof_end(S_LEX_REDUCE, st);
+  ctx->prof_begin(S_LEX_SCAN, st);
+  lex_scan_docs<<<ndocs, LEX_SCAN_BLOCK, 0, st>>>(a);
+  ctx->prof_end(S_LEX_SCAN, st);
+  ctx->prof_begin(S_LEX_EMIT, st);
+  lex_emit<false, true><<<(unsigned)ctiles, LEX_BLOCK, 0, st>>>(a);
+  ctx->prof_end(S_LEX_EMIT, st);
+  ctx->launches += 5;
+  CK(hipGetLastError());
+  return 0;
+}
+
+// Parser arguments common to the single and batched paths.
+void parse_args_init(alpacc_ctx *ctx, PArgs &a) {
+  memset(&a, 0, sizeof(a));
+  a.q = ctx->q;
+  a.k = ctx->k;
+  a.nT = ctx->nT;
+  a.start_t = ctx->start_t;
+  a.end_t = ctx->end_t;
+  a.empty_t = ctx->empty_t;
+  a.index8 = ctx->d_index8;
+  a.index32 = ctx->d_pair_index;
+  a.index_len = (uint32_t)ctx->index_len;
+  a.einfo = ctx->d_einfo;
+  a.E = ctx->E;
+  a.stackw = ctx->d_stackw;
+  a.prodw = ctx->d_prodw;
+  a.nstack = (uint32_t)ctx->nstackw;
+  a.nprod = (uint32_t)ctx->nprodw;
+}
+
+// keys, tile totals and the tile scan (ntiles_max: tiles to cover, the device count in batch)
+template <bool BATCH>
+int parse_phase1(alpacc_ctx *ctx, PArgs &a, uint64_t ntiles_max, hipStream_t st) {
+  ctx->prof_begin(S_PARSE_KEYS, st);
+  const uint64_t nsuper = (ntiles_max + PKK_TILES - 1) / PKK_TILES;
+  const unsigned kgrid =
+      (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(nsuper, (uint64_t)std::max(1, ctx->cus) * 2));
+  if (ctx->d_index8)
+    parse_keys2<true, BATCH><<<kgrid, PKK_THREADS, ctx->index_len, st>>>(a);
+  else
+    parse_keys2<false, BATCH><<<kgrid, PKK_THREADS, 0, st>>>(a);
+  const uint64_t nb = std::max<uint64_t>(1, (ntiles_max + PSB - 1) / PSB);
+  SegTup *btot = (SegTup *)ctx->buf(B_BTOT, (nb + 1) * sizeof(SegTup));
+  if (!btot) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  pscan_reduce<BATCH><<<(unsigned)nb, PSB, 0, st>>>(a, btot);
+  pscan_top<<<1, PSB, 0, st>>>(a, btot, nb);
+  pscan_down<BATCH><<<(unsigned)nb, PSB, 0, st>>>(a, btot);
+  ctx->prof_end(S_PARSE_KEYS, st);
+  ctx->launches += 4;
+  CK(hipGetLastError());
+  return 0;
+}
+
+// element scratch for B brackets and N productions
+struct ElemBufs {
+  int32_t *depth, *dl1, *sx, *sl1;
+  uint16_t *br;
+};
+
+int elem_bufs(alpacc_ctx *ctx, uint64_t B, uint64_t N, ElemBufs &e, hipStream_t st) {
+  const uint64_t nbc = (B + WAVE - 1) / WAVE, npc = (N + WAVE - 1) / WAVE;
+  e.depth = (int32_t *)ctx->buf(B_DEPTH, (B + 16) * sizeof(int32_t));
+  e.br = (uint16_t *)ctx->buf(B_BR, (B + 16) * sizeof(uint16_t));
+  e.dl1 = (int32_t *)ctx->buf(B_DL1, (nbc + 16) * sizeof(int32_t));
+  e.sx = (int32_t *)ctx->buf(B_SX, (N + 16) * sizeof(int32_t));
+  e.sl1 = (int32_t *)ctx->buf(B_SL1, (npc + 16) * sizeof(int32_t));
+  if (!e.depth || !e.br || !e.dl1 || !e.sx || !e.sl1) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  fill_i32<<<grid_for(nbc + 16, 256, 4096), 256, 0, st>>>(e.dl1, nbc + 16, INT32_MAX);
+  fill_i32<<<grid_for(npc + 16, 256, 4096), 256, 0, st>>>(e.sl1, npc + 16, INT32_MAX);
+  ctx->launches += 2;
+  return 0;
+}
+
 template <typename S>
 int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, const S *ends,
                      uint64_t T, uint32_t *parent, uint32_t *node, S *span, uint64_t cap,
@@ -1434,97 +2201,342 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, co
   }
   *num_nodes = 0;
   const uint64_t P = T + 2;
-  uint32_t *flags = (uint32_t *)ctx->buf(B_SMALL, 64) + 8;
-  uint32_t *ent = (uint32_t *)ctx->buf(B_ENT, P * sizeof(uint32_t));
-  I2 *lens = (I2 *)ctx->buf(B_LENS, P * sizeof(I2));
-  I2 *offs = (I2 *)ctx->buf(B_OFFS, P * sizeof(I2));
-  if (!flags || !ent || !lens || !offs) {
+  if (P * (uint64_t)std::max(ctx->max_nb, ctx->max_np) >= (1ull << 31)) {
+    ctx->err = "token stream too long for 32-bit element counts";
+    return ALPACC_ERR_ARG;
+  }
+  const uint64_t ntiles = (P + PK_TILE - 1) / PK_TILE;
+  uint32_t *small = (uint32_t *)ctx->buf(B_SMALL, 64 * sizeof(uint32_t));
+  uint16_t *ent = (uint16_t *)ctx->buf(B_ENT, (ntiles * PK_TILE + 8) * sizeof(uint16_t));
+  PTup *tiles = (PTup *)ctx->buf(B_PTILES, (ntiles + 1) * sizeof(PTup));
+  PTup *pre = (PTup *)ctx->buf(B_PPRE, (ntiles + 1) * sizeof(PTup));
+  if (!small || !ent || !tiles || !pre) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
-  ParseArgs a;
+  uint32_t *flags = small + 8;
+  PTup *total = reinterpret_cast<PTup *>(small + 16);
+  PArgs a;
+  parse_args_init(ctx, a);
   a.kinds = kinds;
   a.T = T;
-  a.q = ctx->q;
-  a.k = ctx->k;
-  a.nT = ctx->nT;
-  a.start_t = ctx->start_t;
-  a.end_t = ctx->end_t;
-  a.empty_t = ctx->empty_t;
-  a.pair_index = ctx->d_pair_index;
-  a.pair_entry = ctx->d_pair_entry;
-  a.stacks = ctx->d_stacks;
-  a.prods = ctx->d_prods;
-  a.arity = ctx->d_arity;
-  a.p2t = ctx->d_p2t;
+  a.P = P;
+  a.ent = ent;
+  a.tiles = tiles;
+  a.pre = pre;
+  a.total = total;
   a.flags = flags;
+  a.ntiles = ntiles;
   ctx->prof_begin(S_PARSE, st);
   clear_u32<<<1, 64, 0, st>>>(flags, 4);
-  parse_keys<<<grid_for(P, 256), 256, 0, st>>>(a, ent, lens);
-  ctx->launches += 2;
-  CK(hipGetLastError());
-  I2 tot;
-  int rc = scan2(ctx, SrcArray{lens}, SinkArray{offs}, P, &tot, st);
+  ctx->launches++;
+  int rc = parse_phase1<false>(ctx, a, ntiles, st);
   if (rc) return rc;
-  CK(hipMemcpyAsync(ctx->h_small, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  CK(hipMemcpyAsync(ctx->h_small, small + 8, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   CK(hipStreamSynchronize(st));
   if (ctx->h_small[0]) return ALPACC_REJECT;  // a missing pair (to_keys, parser.fut:235-240)
-  const uint64_t B = (uint32_t)tot.x, N = (uint32_t)tot.y;
-  uint16_t *br = (uint16_t *)ctx->buf(B_BR, B * sizeof(uint16_t) + 2);
-  uint16_t *prod = (uint16_t *)ctx->buf(B_PROD, N * sizeof(uint16_t) + 2);
-  if (!br || !prod) {
-    ctx->err = "scratch allocation failed";
-    return ALPACC_ERR_HIP;
-  }
-  parse_expand<<<grid_for(P, 256), 256, 0, st>>>(a, ent, offs, br, prod);
+  PTup tot;
+  memcpy(&tot, ctx->h_small + 8, sizeof(PTup));
+  // depths (parser.fut:153-166): a prefix below zero, or a non-zero end
+  if (tot.nb > 0 && (tot.mind < 0 || tot.dd != 0)) return ALPACC_REJECT;
+  const uint64_t B = (uint32_t)tot.nb, N = (uint32_t)tot.np;
+  a.subst = (uint64_t)(uint32_t)tot.nt == T ? 1 : 0;
+  ctx->last_counts[0] = T;
+  ctx->last_counts[1] = P;
+  ctx->last_counts[2] = B;
+  ctx->last_counts[3] = N;
+  *num_nodes = N;
+  const int write_nodes = N <= cap ? 1 : 0;
+  ElemBufs e;
+  rc = elem_bufs(ctx, B, N, e, st);
+  if (rc) return rc;
+  const size_t xlds = 4 * ((ctx->nstackw + 1) & ~(size_t)1) + 8 * ctx->nprodw;
+  ctx->prof_begin(S_PARSE_EXPAND, st);
+  parse_expand2<S, false><<<(unsigned)ntiles, PK_THREADS, xlds, st>>>(
+      a, starts, ends, e.depth, e.br, e.dl1, e.sx, e.sl1, node, span, write_nodes);
+  ctx->prof_end(S_PARSE_EXPAND, st);
   ctx->launches++;
-  // bracket depths and matching
+  CK(hipGetLastError());
   if (B > 0) {
-    int32_t *depth = (int32_t *)ctx->buf(B_DEPTH, (B + 16) * sizeof(int32_t));
-    int32_t *dl1 = (int32_t *)ctx->buf(B_DL1, ((B + 63) / 64 + 16) * sizeof(int32_t));
-    if (!depth || !dl1) {
-      ctx->err = "scratch allocation failed";
-      return ALPACC_ERR_HIP;
-    }
-    rc = scan2(ctx, SrcBracket{br}, SinkDepth{depth, dl1, flags, B}, B, nullptr, st);
-    if (rc) return rc;
     Levels lv;
-    rc = build_levels(ctx, depth, dl1, B, lv, st);
+    rc = build_levels(ctx, e.depth, e.dl1, B, lv, st);
     if (rc) return rc;
-    bracket_match<<<grid_for(B, 256), 256, 0, st>>>(br, B, lv, flags);
+    ctx->prof_begin(S_BRACKET_MATCH, st);
+    bracket_match<false><<<grid_for(B, 256), 256, 0, st>>>(e.br, B, lv, flags, a);
+    ctx->prof_end(S_BRACKET_MATCH, st);
     ctx->launches++;
     CK(hipGetLastError());
   }
-  CK(hipMemcpyAsync(ctx->h_small, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  CK(hipStreamSynchronize(st));
-  if (ctx->h_small[0]) return ALPACC_REJECT;
-  *num_nodes = N;
-  if (N > cap) {
-    ctx->err = "output capacity too small";
-    return ALPACC_ERR_CAPACITY;
+  if (!write_nodes || N == 0) {
+    CK(hipMemcpyAsync(ctx->h_small, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (ctx->h_small[0]) return ALPACC_REJECT;
+    if (!write_nodes) {
+      ctx->err = "output capacity too small";
+      return ALPACC_ERR_CAPACITY;
+    }
+    return ALPACC_ACCEPT;
   }
-  if (N == 0) return ALPACC_ACCEPT;
-  int32_t *sx = (int32_t *)ctx->buf(B_SX, (N + 16) * sizeof(int32_t));
-  int32_t *rank = (int32_t *)ctx->buf(B_RANK, (N + 16) * sizeof(int32_t));
-  int32_t *sl1 = (int32_t *)ctx->buf(B_SL1, ((N + 63) / 64 + 16) * sizeof(int32_t));
-  if (!sx || !rank || !sl1) {
-    ctx->err = "scratch allocation failed";
-    return ALPACC_ERR_HIP;
-  }
-  I2 ptot;
-  rc = scan2(ctx, SrcProd{prod, ctx->d_arity, ctx->d_p2t}, SinkProd{sx, rank, sl1, N}, N, &ptot, st);
-  if (rc) return rc;
-  const int subst = (uint64_t)(uint32_t)ptot.y == T ? 1 : 0;
   Levels lv;
-  rc = build_levels(ctx, sx, sl1, N, lv, st);
+  rc = build_levels(ctx, e.sx, e.sl1, N, lv, st);
   if (rc) return rc;
-  parse_nodes<S><<<grid_for(N, 256), 256, 0, st>>>(a, prod, rank, N, lv, starts, ends, parent,
-                                                   node, span, subst);
+  ctx->prof_begin(S_PARSE_PARENTS, st);
+  parse_parents<false><<<grid_for(N, 256), 256, 0, st>>>(N, lv, parent, a);
+  ctx->prof_end(S_PARSE_PARENTS, st);
   ctx->launches++;
   CK(hipGetLastError());
   ctx->prof_end(S_PARSE, st);
+  CK(hipMemcpyAsync(ctx->h_small, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   CK(hipStreamSynchronize(st));
+  if (ctx->h_small[0]) return ALPACC_REJECT;  // brackets_matches (parser.fut:174-184)
   return ALPACC_ACCEPT;
+}
+
+// Final per-document results of the batched parser.
+__global__ void pb_finish(PArgs a, alpacc_doc_result *out) {
+  const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= a.ndocs) return;
+  const PDoc pd = a.pdocs[d];
+  alpacc_doc_result r;
+  r.status = (pd.status && (a.doc_flags[d] & 2u) == 0) ? ALPACC_ACCEPT : ALPACC_REJECT;
+  r.reserved = 0;
+  r.offset = pd.node_off;
+  r.count = r.status == ALPACC_ACCEPT ? pd.N : 0;
+  out[d] = r;
+}
+
+__global__ void lb_finish(const LexDoc *docs, const uint32_t *doc_res, uint32_t ndocs,
+                          alpacc_doc_result *out) {
+  const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndocs) return;
+  alpacc_doc_result r;
+  r.status = doc_res[2 * d] ? ALPACC_ACCEPT : ALPACC_REJECT;
+  r.reserved = 0;
+  r.offset = docs[d].tok_base;
+  r.count = r.status == ALPACC_ACCEPT ? doc_res[2 * d + 1] : 0;
+  out[d] = r;
+}
+
+// Batched lex + parse of many documents: one lexer pass and one parser pass over all of them,
+// with one host synchronisation (to size the bracket / production scratch).
+int run_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
+                    const uint64_t *lengths, uint32_t ndocs, uint32_t *parent, uint32_t *node,
+                    uint32_t *span, uint64_t cap, uint64_t *total_nodes,
+                    alpacc_doc_result *results, hipStream_t st) {
+  if (!ctx->has_parser) {
+    ctx->err = "bundle has no parser";
+    return ALPACC_ERR_NO_PARSER;
+  }
+  *total_nodes = 0;
+  uint64_t nbytes = 0, tiles_max = 0;
+  for (uint32_t d = 0; d < ndocs; ++d) {
+    nbytes += lengths[d];
+    tiles_max += (lengths[d] + 3 + PK_TILE - 1) / PK_TILE;
+  }
+  const uint64_t tslots = nbytes + ndocs + 1;
+  if (tslots >= (1ull << 32) - 1 || tiles_max >= (1ull << 32) / PK_TILE) {
+    ctx->err = "batch too large: token slots and key positions must stay below 2^32";
+    return ALPACC_ERR_ARG;
+  }
+  uint32_t *kinds = (uint32_t *)ctx->buf(B_TOK_KIND, tslots * 4);
+  uint32_t *starts = (uint32_t *)ctx->buf(B_TOK_START, tslots * 4);
+  uint32_t *ends = (uint32_t *)ctx->buf(B_TOK_END, tslots * 4);
+  uint32_t *doc_res = (uint32_t *)ctx->buf(B_DOC_RES, (2ull * ndocs + 2) * 4);
+  PDoc *pdocs = (PDoc *)ctx->buf(B_PDOCS, (ndocs + 1) * sizeof(PDoc));
+  uint32_t *doc_flags = (uint32_t *)ctx->buf(B_DOC_FLAGS, (ndocs + 1) * 4);
+  uint32_t *ptile_doc = (uint32_t *)ctx->buf(B_PTILE_DOC, (tiles_max + 1) * 4);
+  uint16_t *ent = (uint16_t *)ctx->buf(B_ENT, (tiles_max * PK_TILE + 8) * sizeof(uint16_t));
+  PTup *tiles = (PTup *)ctx->buf(B_PTILES, (tiles_max + 1) * sizeof(PTup));
+  PTup *pre = (PTup *)ctx->buf(B_PPRE, (tiles_max + 1) * sizeof(PTup));
+  alpacc_doc_result *d_results =
+      (alpacc_doc_result *)ctx->buf(B_RESULTS, (ndocs + 1) * sizeof(alpacc_doc_result));
+  if (!kinds || !starts || !ends || !doc_res || !pdocs || !doc_flags || !ptile_doc || !ent ||
+      !tiles || !pre || !d_results) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  LexDoc *ldocs = nullptr;
+  int rc = run_lex_batch(ctx, d_in, offsets, lengths, ndocs, kinds, starts, ends, doc_res, &ldocs,
+                         st);
+  if (rc) return rc;
+  uint32_t *small = (uint32_t *)ctx->buf(B_SMALL, 64 * sizeof(uint32_t));
+  uint64_t *ntiles_dev = reinterpret_cast<uint64_t *>(small + 24);
+  PTup *total = reinterpret_cast<PTup *>(small + 16);
+  PArgs a;
+  parse_args_init(ctx, a);
+  a.kinds = kinds;
+  a.ent = ent;
+  a.tiles = tiles;
+  a.pre = pre;
+  a.total = total;
+  a.flags = small + 8;
+  a.pdocs = pdocs;
+  a.ptile_doc = ptile_doc;
+  a.ntiles_dev = ntiles_dev;
+  a.ndocs = ndocs;
+  a.doc_flags = doc_flags;
+  ctx->prof_begin(S_PARSE, st);
+  pb_setup<<<1, PS_THREADS, 0, st>>>(a, ldocs, doc_res, ntiles_dev);
+  pb_tile_docs<<<ndocs, 256, 0, st>>>(a);
+  ctx->launches += 2;
+  rc = parse_phase1<true>(ctx, a, tiles_max, st);
+  if (rc) return rc;
+  pb_docs<<<(ndocs + 255) / 256, 256, 0, st>>>(a);
+  ctx->launches++;
+  CK(hipGetLastError());
+  CK(hipMemcpyAsync(ctx->h_small, small + 16, 12 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+  PTup tot;
+  memcpy(&tot, ctx->h_small, sizeof(PTup));
+  uint64_t ntiles;
+  memcpy(&ntiles, ctx->h_small + 8, sizeof(uint64_t));
+  if (tot.nb < 0 || tot.np < 0) {
+    ctx->err = "batch too large: more than 2^31 brackets or productions";
+    return ALPACC_ERR_ARG;
+  }
+  const uint64_t B = (uint32_t)tot.nb, N = (uint32_t)tot.np;
+  *total_nodes = N;
+  ctx->last_counts[0] = 0;
+  ctx->last_counts[1] = ntiles * PK_TILE;
+  ctx->last_counts[2] = B;
+  ctx->last_counts[3] = N;
+  const int write_nodes = N <= cap ? 1 : 0;
+  ElemBufs e;
+  rc = elem_bufs(ctx, B, N, e, st);
+  if (rc) return rc;
+  a.chunk_doc = (uint32_t *)ctx->buf(B_CHUNK_DOC, ((N + WAVE - 1) / WAVE + 1) * 4);
+  if (!a.chunk_doc) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  if (ntiles > 0) {
+    const size_t xlds = 4 * ((ctx->nstackw + 1) & ~(size_t)1) + 8 * ctx->nprodw;
+    ctx->prof_begin(S_PARSE_EXPAND, st);
+    parse_expand2<uint32_t, true><<<(unsigned)ntiles, PK_THREADS, xlds, st>>>(
+        a, starts, ends, e.depth, e.br, e.dl1, e.sx, e.sl1, node, span, write_nodes);
+    ctx->prof_end(S_PARSE_EXPAND, st);
+    ctx->launches++;
+  }
+  if (B > 0) {
+    Levels lv;
+    rc = build_levels(ctx, e.depth, e.dl1, B, lv, st);
+    if (rc) return rc;
+    ctx->prof_begin(S_BRACKET_MATCH, st);
+    bracket_match<true><<<grid_for(B, 256), 256, 0, st>>>(e.br, B, lv, a.flags, a);
+    ctx->prof_end(S_BRACKET_MATCH, st);
+    ctx->launches++;
+  }
+  if (write_nodes && N > 0) {
+    Levels lv;
+    rc = build_levels(ctx, e.sx, e.sl1, N, lv, st);
+    if (rc) return rc;
+    ctx->prof_begin(S_PARSE_PARENTS, st);
+    parse_parents<true><<<grid_for(N, 256), 256, 0, st>>>(N, lv, parent, a);
+    ctx->prof_end(S_PARSE_PARENTS, st);
+    ctx->launches++;
+  }
+  pb_finish<<<(ndocs + 255) / 256, 256, 0, st>>>(a, d_results);
+  ctx->launches++;
+  CK(hipGetLastError());
+  ctx->prof_end(S_PARSE, st);
+  CK(hipMemcpyAsync(results, d_results, ndocs * sizeof(alpacc_doc_result), hipMemcpyDeviceToHost,
+                    st));
+  CK(hipStreamSynchronize(st));
+  if (!write_nodes) {
+    ctx->err = "output capacity too small";
+    return ALPACC_ERR_CAPACITY;
+  }
+  return ALPACC_ACCEPT;
+}
+
+// Per pair entry totals and per element relative values (see the parser kernels) from the
+// pair table, stacks and productions of the generated bundle (Encode.hs:336-375 layout restated
+// in alpacc_amd/gen/tables.py).
+bool build_parser_tables(alpacc_ctx *ctx, const alpacc_tables *t, std::string &err) {
+  const uint32_t E = t->num_pair_entries;
+  if (E == 0 || E >= 65535 || t->num_productions >= 65535 || t->num_terminals >= 65535 ||
+      t->q + t->k > (uint32_t)PK_MAXW || t->q + t->k == 0) {
+    err = "parser tables out of range";
+    return false;
+  }
+  std::vector<EntInfo> info(E);
+  std::vector<uint32_t> stackw;
+  std::vector<uint2> prodw;
+  for (uint32_t e = 0; e < E; ++e) {
+    const uint32_t *e4 = t->pair_entry + 4ull * e;
+    if (e4[1] < e4[0] || e4[3] < e4[2] || e4[1] > t->stacks_len || e4[3] > t->productions_len) {
+      err = "pair entry out of range";
+      return false;
+    }
+    EntInfo &f = info[e];
+    f.nb = (uint16_t)(e4[1] - e4[0]);
+    f.np = (uint16_t)(e4[3] - e4[2]);
+    f.sst = (uint16_t)stackw.size();
+    f.pst = (uint16_t)prodw.size();
+    int32_t r = 0, mind = INT16_MAX;
+    for (uint32_t s = e4[0]; s < e4[1]; ++s) {
+      const uint16_t b = t->stacks[s];
+      const bool left = (b & 0x8000u) != 0;
+      r += left ? 1 : -1;
+      mind = std::min(mind, r);
+      const int32_t rel = r - (left ? 1 : 0);  // depths: scan - [is_left]
+      stackw.push_back((uint32_t)b | ((uint32_t)(uint16_t)(int16_t)rel << 16));
+    }
+    f.dd = (int16_t)r;
+    f.mind = (int16_t)mind;
+    int32_t ds = 0, nt = 0;
+    for (uint32_t s = e4[2]; s < e4[3]; ++s) {
+      const uint16_t pr = t->productions[s];
+      if (pr >= t->num_productions) {
+        err = "production id out of range";
+        return false;
+      }
+      const int32_t term = t->prod_to_terminal[pr];
+      uint2 w;
+      w.x = (uint32_t)pr | ((uint32_t)(term + 1) << 16);
+      w.y = (uint32_t)(uint16_t)(int16_t)ds | ((uint32_t)nt << 16);
+      prodw.push_back(w);
+      ds += t->arities[pr] - 1;
+      nt += term >= 0 ? 1 : 0;
+    }
+    f.ds = (int16_t)ds;
+    f.nt = (uint16_t)nt;
+    ctx->max_nb = std::max<uint32_t>(ctx->max_nb, f.nb);
+    ctx->max_np = std::max<uint32_t>(ctx->max_np, f.np);
+    if (stackw.size() >= 65535 || prodw.size() >= 65535 || std::abs(r) > 30000 ||
+        std::abs(ds) > 30000) {
+      err = "pair entries too long";
+      return false;
+    }
+  }
+  for (uint64_t i = 0; i < t->pair_index_len; ++i) {
+    if (t->pair_index[i] != 0xFFFFFFFFu && t->pair_index[i] >= E) {
+      err = "pair index entry out of range";
+      return false;
+    }
+  }
+  uint64_t want_len = 1;
+  for (uint32_t j = 0; j < t->q + t->k; ++j) want_len *= t->num_terminals;
+  if (want_len != t->pair_index_len) {
+    err = "pair index length is not num_terminals^(q+k)";
+    return false;
+  }
+  if (stackw.empty()) stackw.push_back(0);
+  if (prodw.empty()) prodw.push_back(make_uint2(0, 0));
+  ctx->index_len = t->pair_index_len;
+  ctx->d_pair_index = upload(t->pair_index, t->pair_index_len, err);
+  if (E < 255 && t->pair_index_len <= (uint64_t)PK_IDX8_MAX) {
+    std::vector<uint8_t> idx8(t->pair_index_len);
+    for (uint64_t i = 0; i < t->pair_index_len; ++i)
+      idx8[i] = t->pair_index[i] == 0xFFFFFFFFu ? (uint8_t)0xFF : (uint8_t)t->pair_index[i];
+    ctx->d_index8 = upload(idx8.data(), idx8.size(), err);
+  }
+  ctx->nstackw = stackw.size();
+  ctx->nprodw = prodw.size();
+  ctx->d_einfo = upload(info.data(), info.size(), err);
+  ctx->d_stackw = upload(stackw.data(), stackw.size(), err);
+  ctx->d_prodw = upload(prodw.data(), prodw.size(), err);
+  return err.empty();
 }
 
 }  // namespace
@@ -1545,18 +2557,18 @@ alpacc_ctx *alpacc_create(const alpacc_tables *t, int device) {
   }
   alpacc_ctx *ctx = new alpacc_ctx();
   ctx->device = device;
-  hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device);
+  (void)hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device);
   std::string &err = g_create_error;
   if (hipHostMalloc((void **)&ctx->h_small, 64, hipHostMallocDefault) != hipSuccess) {
     err = "hipHostMalloc failed";
-    delete ctx;
+    alpacc_destroy(ctx);
     return nullptr;
   }
   if (t->num_endos > 0) {
     if (t->num_endos > 65535 || t->num_states > 256 || t->num_classes > 256 ||
         t->init_state >= t->num_states) {
       err = "lexer tables out of range";
-      delete ctx;
+      alpacc_destroy(ctx);
       return nullptr;
     }
     ctx->has_lexer = true;
@@ -1573,12 +2585,12 @@ alpacc_ctx *alpacc_create(const alpacc_tables *t, int device) {
     ctx->d_dfa = upload(t->dfa_step, (size_t)ctx->m1 * ctx->C, err);
     ctx->d_accept = upload(t->state_accept, ctx->m1, err);
     if (!err.empty()) {
-      delete ctx;
+      alpacc_destroy(ctx);
       return nullptr;
     }
     if ((size_t)ctx->m1 * ctx->C > (size_t)LEX_MAX_DFA) {
       err = "lexer DFA table too large";
-      delete ctx;
+      alpacc_destroy(ctx);
       return nullptr;
     }
     size_t lds = (size_t)ctx->e * ctx->C * 2 + 256;
@@ -1588,15 +2600,17 @@ alpacc_ctx *alpacc_create(const alpacc_tables *t, int device) {
     if (ctx->comp_in_lds) lds += comp_bytes;
     if (lds > 160 * 1024) {
       err = "lexer tables do not fit in LDS";
-      delete ctx;
+      alpacc_destroy(ctx);
       return nullptr;
     }
     if (lds > 64 * 1024) {
       const int want = (int)lds;
-      if (hipFuncSetAttribute((const void *)lex_reduce<false>,
+      if (hipFuncSetAttribute((const void *)lex_reduce<false, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess ||
+          hipFuncSetAttribute((const void *)lex_reduce<false, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess) {
         err = "lexer tables do not fit in LDS";
-        delete ctx;
+        alpacc_destroy(ctx);
         return nullptr;
       }
     }
@@ -1610,15 +2624,9 @@ alpacc_ctx *alpacc_create(const alpacc_tables *t, int device) {
     ctx->start_t = t->start_terminal;
     ctx->end_t = t->end_terminal;
     ctx->empty_t = t->empty_terminal;
-    ctx->P = t->num_productions;
-    ctx->d_pair_index = upload(t->pair_index, t->pair_index_len, err);
-    ctx->d_pair_entry = upload(t->pair_entry, (size_t)t->num_pair_entries * 4, err);
-    ctx->d_stacks = upload(t->stacks, t->stacks_len, err);
-    ctx->d_prods = upload(t->productions, t->productions_len, err);
-    ctx->d_arity = upload(t->arities, t->num_productions, err);
-    ctx->d_p2t = upload(t->prod_to_terminal, t->num_productions, err);
-    if (!err.empty()) {
-      delete ctx;
+    ctx->E = t->num_pair_entries;
+    if (!build_parser_tables(ctx, t, err)) {
+      alpacc_destroy(ctx);
       return nullptr;
     }
   }
@@ -1630,9 +2638,9 @@ void alpacc_destroy(alpacc_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   for (auto &s : ctx->scratch)
     if (s.first) (void)hipFree(s.first);
-  void *tabs[] = {ctx->d_cls,          ctx->d_step,         ctx->d_comp,   ctx->d_apply,
-                  ctx->d_dfa,          ctx->d_accept,       ctx->d_pair_index, ctx->d_pair_entry,
-                  ctx->d_stacks,       ctx->d_prods,        ctx->d_arity,  ctx->d_p2t};
+  void *tabs[] = {ctx->d_cls,    ctx->d_step,     ctx->d_comp,   ctx->d_apply,
+                  ctx->d_dfa,    ctx->d_accept,   ctx->d_pair_index, ctx->d_index8,
+                  ctx->d_einfo,  ctx->d_stackw,   ctx->d_prodw};
   for (void *p : tabs)
     if (p) (void)hipFree(p);
   if (ctx->h_small) (void)hipHostFree(ctx->h_small);
@@ -1645,6 +2653,12 @@ void alpacc_destroy(alpacc_ctx *ctx) {
 const char *alpacc_last_error(alpacc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int alpacc_last_launch_count(alpacc_ctx *ctx) { return ctx ? ctx->launches : 0; }
+
+int alpacc_last_counts(alpacc_ctx *ctx, uint64_t *counts) {
+  if (!ctx || !counts) return ALPACC_ERR_ARG;
+  for (int i = 0; i < 4; ++i) counts[i] = ctx->last_counts[i];
+  return 0;
+}
 
 int alpacc_profile_enable(alpacc_ctx *ctx, int on) {
   if (!ctx) return ALPACC_ERR_ARG;
@@ -1802,6 +2816,63 @@ int alpacc_parse_chunked(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint6
   return rc;
 }
 
+int alpacc_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
+                     const uint64_t *lengths, uint32_t num_docs, uint32_t *d_kinds,
+                     uint32_t *d_starts, uint32_t *d_ends, uint64_t cap,
+                     alpacc_doc_result *results, void *stream) {
+  if (!ctx || !offsets || !lengths || !results || !d_kinds || !d_starts || !d_ends || num_docs == 0)
+    return ALPACC_ERR_ARG;
+  uint64_t need = 0;
+  for (uint32_t d = 0; d < num_docs; ++d) need += lengths[d] + 1;
+  if (cap < need) {
+    ctx->err = "token capacity must be at least the sum of (length + 1)";
+    return ALPACC_ERR_CAPACITY;
+  }
+  ctx->launches = 0;
+  ctx->err.clear();
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  uint32_t *doc_res = (uint32_t *)ctx->buf(B_DOC_RES, (2ull * num_docs + 2) * 4);
+  alpacc_doc_result *d_results =
+      (alpacc_doc_result *)ctx->buf(B_RESULTS, (num_docs + 1) * sizeof(alpacc_doc_result));
+  if (!doc_res || !d_results) {
+    ctx->err = "scratch allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  LexDoc *ldocs = nullptr;
+  int rc = run_lex_batch(ctx, d_in, offsets, lengths, num_docs, d_kinds, d_starts, d_ends, doc_res,
+                         &ldocs, st);
+  if (rc == 0) {
+    lb_finish<<<(num_docs + 255) / 256, 256, 0, st>>>(ldocs, doc_res, num_docs, d_results);
+    ctx->launches++;
+    if (hipMemcpyAsync(results, d_results, num_docs * sizeof(alpacc_doc_result),
+                       hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      ctx->err = "HIP error in the batched lexer";
+      rc = ALPACC_ERR_HIP;
+    } else {
+      rc = ALPACC_ACCEPT;
+    }
+  }
+  if (ctx->prof) ctx->prof_collect();
+  return rc;
+}
+
+int alpacc_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
+                       const uint64_t *lengths, uint32_t num_docs, uint32_t *d_parent,
+                       uint32_t *d_node, uint32_t *d_span, uint64_t cap, uint64_t *total_nodes,
+                       alpacc_doc_result *results, void *stream) {
+  if (!ctx || !offsets || !lengths || !results || !total_nodes || num_docs == 0)
+    return ALPACC_ERR_ARG;
+  ctx->launches = 0;
+  ctx->err.clear();
+  (void)hipSetDevice(ctx->device);
+  const int rc = run_parse_batch(ctx, d_in, offsets, lengths, num_docs, d_parent, d_node, d_span,
+                                 cap, total_nodes, results, (hipStream_t)stream);
+  if (ctx->prof) ctx->prof_collect();
+  return rc;
+}
+
 int alpacc_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -1818,6 +2889,16 @@ void *alpacc_device_alloc(uint64_t bytes) {
 
 void alpacc_device_free(void *ptr) {
   if (ptr) (void)hipFree(ptr);
+}
+
+void *alpacc_host_alloc(uint64_t bytes) {
+  void *p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+  return p;
+}
+
+void alpacc_host_free(void *ptr) {
+  if (ptr) (void)hipHostFree(ptr);
 }
 
 int alpacc_copy_to_device(void *dst, const void *src, uint64_t bytes, void *stream) {

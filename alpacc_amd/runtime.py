@@ -59,7 +59,8 @@ EXPORTS = ("alpacc_create", "alpacc_create_error", "alpacc_destroy", "alpacc_las
            "alpacc_stream_create", "alpacc_stream_destroy", "alpacc_stream_sync",
            "alpacc_event_create", "alpacc_event_destroy", "alpacc_event_record",
            "alpacc_event_elapsed_ms", "alpacc_profile_enable", "alpacc_profile_read",
-           "alpacc_lex_chunked", "alpacc_parse_chunked")
+           "alpacc_lex_chunked", "alpacc_parse_chunked", "alpacc_lex_batch", "alpacc_parse_batch",
+           "alpacc_last_counts", "alpacc_host_alloc", "alpacc_host_free")
 
 # the chunk size of the reference's generated entries (Generator.hs:24-76: lexer.lex_int 16777216)
 REFERENCE_CHUNK = 16777216
@@ -90,6 +91,13 @@ def load_library():
     L.alpacc_parse_tokens.argtypes = [vp, vp, vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
     L.alpacc_lex_chunked.argtypes = [vp, vp, u64, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
     L.alpacc_parse_chunked.argtypes = [vp, vp, u64, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
+    L.alpacc_last_counts.argtypes = [vp, vp]
+    L.alpacc_host_alloc.argtypes = [u64]
+    L.alpacc_host_alloc.restype = vp
+    L.alpacc_host_free.argtypes = [vp]
+    L.alpacc_lex_batch.argtypes = [vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp, u64, vp, vp]
+    L.alpacc_parse_batch.argtypes = [vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp, u64,
+                                     ctypes.POINTER(u64), vp, vp]
     L.alpacc_device_count.restype = ctypes.c_int
     L.alpacc_set_device.argtypes = [ctypes.c_int]
     L.alpacc_device_alloc.argtypes = [u64]
@@ -111,6 +119,10 @@ def load_library():
                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64), ctypes.c_int]
     _lib = L
     return L
+
+
+# alpacc_doc_result: status, reserved, offset, count
+DOC_RESULT = np.dtype([("status", "<i4"), ("reserved", "<u4"), ("offset", "<u8"), ("count", "<u8")])
 
 
 class DeviceBuffer:
@@ -144,6 +156,31 @@ class DeviceBuffer:
     def free(self):
         if self.ptr:
             self.lib.alpacc_device_free(ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HostBuffer:
+    """Pinned host memory (for asynchronous copies), viewed as a numpy uint8 array."""
+
+    def __init__(self, nbytes: int, lib=None):
+        self.lib = lib or load_library()
+        self.nbytes = int(nbytes)
+        p = self.lib.alpacc_host_alloc(max(self.nbytes, 1))
+        if not p:
+            raise MemoryError(f"pinned host allocation of {nbytes} bytes failed")
+        self.ptr = p
+        self.array = np.ctypeslib.as_array((ctypes.c_uint8 * max(self.nbytes, 1)).from_address(p))
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            self.lib.alpacc_host_free(ctypes.c_void_p(self.ptr))
             self.ptr = 0
 
     def __del__(self):
@@ -269,12 +306,18 @@ class Context:
 
     def read_profile(self) -> dict:
         """{stage: (total ms, calls)} accumulated since the last read."""
-        m = 8
+        m = 16
         names = (ctypes.c_char_p * m)()
         ms = (ctypes.c_double * m)()
         calls = (ctypes.c_uint64 * m)()
         k = self.lib.alpacc_profile_read(self.handle, names, ms, calls, m)
         return {names[i].decode(): (ms[i], calls[i]) for i in range(k)}
+
+    def last_counts(self) -> dict:
+        """Element counts of the last parser call: tokens, positions, brackets, productions."""
+        out = np.zeros(4, dtype=np.uint64)
+        self.lib.alpacc_last_counts(self.handle, out.ctypes.data)
+        return dict(zip(("tokens", "positions", "brackets", "productions"), (int(x) for x in out)))
 
     # -- device-level calls -----------------------------------------------------------------
     def lex_device(self, d_in: int, n: int, kinds: int, starts: int, ends: int, cap: int,
@@ -404,6 +447,99 @@ class Context:
         nn = nn.value
         return (pb.download(np.uint32, nn, self.stream), nb.download(np.uint32, nn, self.stream),
                 spb.download(np.uint64, 2 * nn, self.stream).reshape(-1, 2))
+
+    # -- batched entries ---------------------------------------------------------------------
+    def upload_batch(self, docs) -> tuple[DeviceBuffer, np.ndarray, np.ndarray]:
+        """Concatenate documents into one device buffer; returns (buffer, offsets, lengths)."""
+        lengths = np.array([len(d) for d in docs], dtype=np.uint64)
+        offsets = np.zeros(len(docs), dtype=np.uint64)
+        if len(docs) > 1:
+            offsets[1:] = np.cumsum(lengths[:-1])
+        blob = np.frombuffer(b"".join(bytes(d) for d in docs), dtype=np.uint8)
+        buf = self.buffer("batch", max(int(blob.shape[0]), 1))
+        if blob.shape[0]:
+            buf.upload(blob, self.stream)
+        return buf, offsets, lengths
+
+    def lex_batch_device(self, d_in: int, offsets: np.ndarray, lengths: np.ndarray, kinds: int,
+                         starts: int, ends: int, cap: int, stream=None) -> np.ndarray:
+        """``alpacc_lex_batch`` on device pointers; returns the per-document results."""
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        lengths = np.ascontiguousarray(lengths, dtype=np.uint64)
+        res = np.zeros(len(lengths), dtype=DOC_RESULT)
+        rc = self.lib.alpacc_lex_batch(self.handle, ctypes.c_void_p(d_in), offsets.ctypes.data,
+                                       lengths.ctypes.data, len(lengths), ctypes.c_void_p(kinds),
+                                       ctypes.c_void_p(starts), ctypes.c_void_p(ends), cap,
+                                       res.ctypes.data, stream)
+        if rc < 0:
+            raise self._err(rc)
+        return res
+
+    def parse_batch_device(self, d_in: int, offsets: np.ndarray, lengths: np.ndarray, parent: int,
+                           node: int, span: int, cap: int, stream=None):
+        """``alpacc_parse_batch`` on device pointers; returns (status, total nodes, results);
+        status -3 means the capacity was too small and total nodes is the size needed."""
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        lengths = np.ascontiguousarray(lengths, dtype=np.uint64)
+        res = np.zeros(len(lengths), dtype=DOC_RESULT)
+        tn = ctypes.c_uint64(0)
+        rc = self.lib.alpacc_parse_batch(self.handle, ctypes.c_void_p(d_in), offsets.ctypes.data,
+                                         lengths.ctypes.data, len(lengths), ctypes.c_void_p(parent),
+                                         ctypes.c_void_p(node), ctypes.c_void_p(span), cap,
+                                         ctypes.byref(tn), res.ctypes.data, stream)
+        if rc < 0 and rc != -3:
+            raise self._err(rc)
+        return rc, tn.value, res
+
+    def lex_int_batch(self, docs) -> list:
+        """``lex_int`` of every document in one batched pass: a list of None / (kinds, starts,
+        ends)."""
+        buf, offsets, lengths = self.upload_batch(docs)
+        cap = int(lengths.sum()) + len(docs) + 1
+        kb = self.buffer("bkinds", 4 * cap)
+        sb = self.buffer("bstarts", 4 * cap)
+        eb = self.buffer("bends", 4 * cap)
+        res = self.lex_batch_device(buf.ptr, offsets, lengths, kb.ptr, sb.ptr, eb.ptr, cap,
+                                    self.stream)
+        k = kb.download(np.uint32, cap, self.stream)
+        s_ = sb.download(np.uint32, cap, self.stream)
+        e = eb.download(np.uint32, cap, self.stream)
+        out = []
+        for r in res:
+            if r["status"] != ACCEPT:
+                out.append(None)
+                continue
+            o, c = int(r["offset"]), int(r["count"])
+            out.append((k[o:o + c], s_[o:o + c], e[o:o + c]))
+        return out
+
+    def parse_int_batch(self, docs, cap: int | None = None) -> list:
+        """``parse_int`` of every document in one batched pass: a list of None /
+        (parent, node, span[N,2])."""
+        buf, offsets, lengths = self.upload_batch(docs)
+        if cap is None:
+            cap = max(16, 4 * int(lengths.sum()) + 16)
+        while True:
+            pb = self.buffer("bparent", 4 * cap)
+            nb = self.buffer("bnode", 4 * cap)
+            spb = self.buffer("bspan", 8 * cap)
+            rc, total, res = self.parse_batch_device(buf.ptr, offsets, lengths, pb.ptr, nb.ptr,
+                                                     spb.ptr, cap, self.stream)
+            if rc == -3:
+                cap = total
+                continue
+            break
+        par = pb.download(np.uint32, total, self.stream)
+        nd = nb.download(np.uint32, total, self.stream)
+        sp = spb.download(np.uint32, 2 * total, self.stream).reshape(-1, 2)
+        out = []
+        for r in res:
+            if r["status"] != ACCEPT:
+                out.append(None)
+                continue
+            o, c = int(r["offset"]), int(r["count"])
+            out.append((par[o:o + c], nd[o:o + c], sp[o:o + c]))
+        return out
 
     def parse(self, data, chunk_size: int | None = None):
         r = self.parse_int(data, chunk_size=chunk_size)

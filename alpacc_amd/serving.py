@@ -1,0 +1,289 @@
+"""Serving loop for lex+parse requests: continuous batching over device-resident tables.
+
+The reference has no serving layer (its runtime is one entry call per document,
+src/Alpacc/Generator/Futhark/Generator.hs:35-44). This loop keeps one grammar's tables on the GPU
+and serves a stream of documents:
+
+* every worker owns a context (tables in HBM, grow-only scratch arenas), a non-blocking stream and
+  pinned staging buffers;
+* a worker that becomes free takes the requests waiting at that moment, in arrival order, as long
+  as their estimated HBM footprint fits the shared budget (admission by footprint) and the batch
+  limits hold, and runs them as one ``alpacc_parse_batch`` call;
+* requests that arrive while batches run wait for the next free worker, so several batches overlap
+  on the device (one per worker stream) and arrivals are coalesced into the next launch;
+* each request gets exactly what ``parse_int`` gives it alone: the batched entry processes every
+  document on its own (tests/test_gpu_serving.py checks alone vs in the mix).
+
+``serve_plain`` is the comparison: one ``alpacc_parse`` call per request in arrival order.
+"""
+from __future__ import annotations
+
+import threading
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .runtime import ACCEPT, Context, DeviceBuffer, HostBuffer, load_library
+
+# Device bytes per input byte while a request is in flight: the input, the token slots
+# (3 x u32 per byte), key positions and pair entries, brackets / productions with their scan
+# values and minima, and the tree (u32 parent, node, span x2) at up to one node per byte.
+FOOTPRINT_PER_BYTE = 48
+FOOTPRINT_FIXED = 1 << 20
+
+
+_CREATE_LOCK = threading.Lock()  # alpacc_create reports errors through one global string
+
+
+def footprint(nbytes: int) -> int:
+    return FOOTPRINT_PER_BYTE * nbytes + FOOTPRINT_FIXED
+
+
+@dataclass
+class Request:
+    id: int
+    data: np.ndarray
+    arrival: float = 0.0
+    status: int = -99
+    tree: tuple | None = None       # (parent, node, span[N, 2]) when accepted
+    t_done: float = 0.0
+    batch: int = -1                 # which launch served it
+    done: threading.Event = field(default_factory=threading.Event)
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.arrival
+
+
+class Server:
+    """Continuous batching of parse requests on one GPU."""
+
+    def __init__(self, bundle, workers: int = 2, hbm_budget: float = 200e9,
+                 max_batch_docs: int = 4096, max_batch_bytes: int = 1 << 30, device: int = 0):
+        self.bundle = bundle
+        self.budget = int(hbm_budget)
+        self.max_docs = max_batch_docs
+        self.max_bytes = max_batch_bytes
+        self.device = device
+        self._cv = threading.Condition()
+        self._pending: deque = deque()
+        self._inflight = 0
+        self._closing = False
+        self._next_id = 0
+        self._launches = 0
+        self.batch_sizes: list = []
+        self._threads = [threading.Thread(target=self._worker, args=(w,), daemon=True)
+                         for w in range(workers)]
+        self._ready = threading.Barrier(workers + 1)
+        self._errors: list = []
+        for t in self._threads:
+            t.start()
+        self._ready.wait()
+        if self._errors:
+            raise self._errors[0]
+
+    # -- client side ----------------------------------------------------------------------------
+    def submit(self, data, arrival: float | None = None) -> Request:
+        arr = np.frombuffer(bytes(data), dtype=np.uint8) if isinstance(data, (bytes, bytearray)) \
+            else np.ascontiguousarray(data, dtype=np.uint8)
+        if footprint(arr.shape[0]) > self.budget:
+            raise ValueError("request larger than the HBM budget")
+        with self._cv:
+            r = Request(self._next_id, arr, time.perf_counter() if arrival is None else arrival)
+            self._next_id += 1
+            self._pending.append(r)
+            self._cv.notify()
+        return r
+
+    def close(self):
+        with self._cv:
+            self._closing = True
+            self._cv.notify_all()
+        for t in self._threads:
+            t.join()
+
+    # -- worker side ----------------------------------------------------------------------------
+    def _admit(self) -> list:
+        """Requests waiting now, in arrival order, while the footprint and batch limits hold."""
+        batch, nbytes = [], 0
+        while self._pending and len(batch) < self.max_docs:
+            r = self._pending[0]
+            n = r.data.shape[0]
+            fp = footprint(n)
+            if batch and (nbytes + n > self.max_bytes or self._inflight + fp > self.budget):
+                break
+            if not batch and self._inflight + fp > self.budget:
+                break  # wait for memory to free up
+            self._pending.popleft()
+            batch.append(r)
+            nbytes += n
+            self._inflight += fp
+        return batch
+
+    def _worker(self, w: int):
+        try:
+            lib = load_library()
+            with _CREATE_LOCK:
+                ctx = Context(self.bundle, self.device)
+            stream = lib.alpacc_stream_create()
+            ctx.stream = stream
+            state = {"in": None, "pin_in": None, "out": None, "pin_out": None, "cap": 0,
+                     "ratio": 1.0}
+        except Exception as exc:  # reported to the constructor
+            self._errors.append(exc)
+            self._ready.wait()
+            return
+        self._ready.wait()
+        while True:
+            with self._cv:
+                while not self._pending and not self._closing:
+                    self._cv.wait()
+                if not self._pending and self._closing:
+                    break
+                batch = self._admit()
+                if not batch:
+                    self._cv.wait(0.001)
+                    continue
+                launch = self._launches
+                self._launches += 1
+                self.batch_sizes.append(len(batch))
+            try:
+                self._run(ctx, lib, stream, state, batch, launch)
+            except Exception as exc:  # fail the batch's requests, keep serving
+                self._errors.append(exc)
+                now = time.perf_counter()
+                for r in batch:
+                    if not r.done.is_set():
+                        r.status = -2
+                        r.t_done = now
+                        r.done.set()
+            finally:
+                with self._cv:
+                    self._inflight -= sum(footprint(r.data.shape[0]) for r in batch)
+                    self._cv.notify_all()
+        for k in ("in", "out"):
+            for b in (state[k] or []):
+                b.free()
+        for k in ("pin_in", "pin_out"):
+            if state[k] is not None:
+                state[k].free()
+        ctx.close()
+        lib.alpacc_stream_destroy(stream)
+
+    @staticmethod
+    def _grow_dev(state, key, sizes, lib):
+        bufs = state[key]
+        if bufs is None or any(b.nbytes < s for b, s in zip(bufs, sizes)):
+            for b in bufs or []:
+                b.free()
+            state[key] = [DeviceBuffer(max(s, 256), lib) for s in sizes]
+        return state[key]
+
+    @staticmethod
+    def _grow_pin(state, key, size, lib):
+        b = state[key]
+        if b is None or b.nbytes < size:
+            if b is not None:
+                b.free()
+            state[key] = HostBuffer(max(size, 256), lib)
+        return state[key]
+
+    def _run(self, ctx, lib, stream, state, batch, launch):
+        lengths = np.array([r.data.shape[0] for r in batch], dtype=np.uint64)
+        offsets = np.zeros(len(batch), dtype=np.uint64)
+        if len(batch) > 1:
+            offsets[1:] = np.cumsum(lengths[:-1])
+        total = int(lengths.sum())
+        pin = self._grow_pin(state, "pin_in", total, lib)
+        for r, o in zip(batch, offsets.tolist()):
+            pin.array[o:o + r.data.shape[0]] = r.data
+        (d_in,) = self._grow_dev(state, "in", [total], lib)
+        if total and lib.alpacc_copy_to_device(d_in.ptr, pin.ptr, total, stream) != 0:
+            raise RuntimeError("host to device copy failed")
+        cap = max(16, int(total * state["ratio"] * 1.25) + 16)
+        while True:
+            outs = self._grow_dev(state, "out", [4 * cap, 4 * cap, 8 * cap], lib)
+            rc, nodes, res = ctx.parse_batch_device(d_in.ptr, offsets, lengths, outs[0].ptr,
+                                                    outs[1].ptr, outs[2].ptr, cap, stream)
+            if rc == -3:
+                cap = int(nodes * 1.25) + 16
+                continue
+            break
+        if total:
+            state["ratio"] = max(state["ratio"] * 0.9, nodes / total)
+        pout = self._grow_pin(state, "pin_out", 16 * max(nodes, 1), lib)
+        if nodes:
+            for k, b in enumerate(outs):
+                width = 8 if k == 2 else 4
+                off = 0 if k == 0 else (4 * nodes if k == 1 else 8 * nodes)
+                if lib.alpacc_copy_to_host(pout.ptr + off, b.ptr, width * nodes, stream) != 0:
+                    raise RuntimeError("device to host copy failed")
+        par = pout.array[:4 * nodes].view(np.uint32)
+        nd = pout.array[4 * nodes:8 * nodes].view(np.uint32)
+        sp = pout.array[8 * nodes:16 * nodes].view(np.uint32).reshape(-1, 2)
+        now = time.perf_counter()
+        for r, x in zip(batch, res):
+            r.status = int(x["status"])
+            if r.status == ACCEPT:
+                o, c = int(x["offset"]), int(x["count"])
+                r.tree = (par[o:o + c].copy(), nd[o:o + c].copy(), sp[o:o + c].copy())
+            r.batch = launch
+            r.t_done = now
+            r.done.set()
+
+
+def serve_plain(ctx: Context, requests, arrivals) -> None:
+    """One alpacc_parse call per request, in arrival order (the comparison for the server)."""
+    lib = ctx.lib
+    t0 = time.perf_counter()
+    for r, a in zip(requests, arrivals):
+        target = t0 + a
+        now = time.perf_counter()
+        if target > now:
+            time.sleep(target - now)
+        r.arrival = target
+        res = ctx.parse_int(r.data)
+        r.status = ACCEPT if res is not None else 0
+        r.tree = res
+        r.t_done = time.perf_counter()
+        r.done.set()
+    lib.alpacc_stream_sync(None)
+
+
+def serve_poisson(server: Server, datas, arrivals) -> list:
+    """Submit every document at its arrival offset (seconds from now); wait for all of them."""
+    reqs = []
+    t0 = time.perf_counter()
+    for d, a in zip(datas, arrivals):
+        target = t0 + a
+        now = time.perf_counter()
+        if target > now:
+            time.sleep(target - now)
+        reqs.append(server.submit(d, arrival=target))
+    for r in reqs:
+        r.done.wait()
+    return reqs
+
+
+def poisson_arrivals(count: int, rate: float, seed: int = 42) -> np.ndarray:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return np.cumsum(rng.exponential(1.0 / rate, size=count))
+
+
+def request_sizes(count: int, lo: int = 256, hi: int = 64 << 20, seed: int = 42) -> np.ndarray:
+    """Log-uniform sizes in [lo, hi]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size=count)).astype(np.int64)
+
+
+def latency_stats(reqs) -> dict:
+    lat = np.array([r.latency for r in reqs]) * 1e3
+    span = max(r.t_done for r in reqs) - min(r.arrival for r in reqs)
+    nbytes = sum(int(r.data.shape[0]) for r in reqs)
+    return {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "mean_ms": round(float(lat.mean()), 3),
+            "GBps": round(nbytes / span / 1e9, 3), "requests": len(reqs), "bytes": nbytes,
+            "seconds": round(span, 3)}

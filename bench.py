@@ -6,9 +6,12 @@ Workload (default): grammars/json.alp, one seeded 256 MiB synthetic JSON documen
 before the timed region. A step is one ``alpacc_parse`` call over the document (it ends with the
 verdict and node count back on the host).
 
-Printed: one JSON line with the metric, the device-time roofline of the dominant kernel and a CPU
-baseline (the C/OpenMP restatement of futhark/lexer.fut + futhark/parser.fut timed on this host
-on a bounded sample of the same workload).
+Printed: one JSON line with the metric, the device-time roofline of the dominant kernel (HIP
+events around each kernel on the stream it runs on), the whole pipeline against its algorithmic
+bytes, a CPU baseline (the C/OpenMP restatement of futhark/lexer.fut + futhark/parser.fut timed on
+this host on a bounded sample of the same workload) and, unless --no-batch, the batched form:
+grammars/sexp.alp over 4096 seeded documents totalling 2 GiB (log-normal lengths, median 64 KiB,
+1 KiB..16 MiB) in one alpacc_parse_batch call against 4096 alpacc_parse calls.
 """
 from __future__ import annotations
 
@@ -35,6 +38,14 @@ def parse_args():
     ap.add_argument("--cpu-sample-mib", type=float, default=16.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verify", action="store_true", help="check the tree against the C oracle")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched sexp comparison")
+    ap.add_argument("--batch-docs", type=int, default=4096)
+    ap.add_argument("--batch-total-mib", type=int, default=2048)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving comparison")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-rate", type=float, default=400.0, help="offered load, requests/s")
+    ap.add_argument("--serve-workers", type=int, default=3)
     return ap.parse_args()
 
 
@@ -64,6 +75,124 @@ def cpu_baseline(bundle, workload, mib, seed):
             "kind": "port",
             "sample": f"{mib:g} MiB seeded JSON document (seed {seed + 1}), {workload} with the "
                       f"C/OpenMP restatement of lexer.fut+parser.fut, mean of {reps} runs"}
+
+
+def batched(args, lib):
+    """The batched form against one call per document, on the same resident documents."""
+    import numpy as np
+    from alpacc_amd import datagen
+    from alpacc_amd.gen.tables import generate
+    from alpacc_amd.runtime import Context, DeviceBuffer
+    bundle = generate(os.path.join(ROOT, "alpacc_amd", "grammars", "sexp.alp"))
+    total = args.batch_total_mib << 20
+    sizes = datagen.sexp_batch_sizes(count=args.batch_docs, total=total)
+    log(f"generating {args.batch_docs} sexp documents, {total >> 20} MiB")
+    d_in = DeviceBuffer(total, lib)
+    offsets = np.zeros(len(sizes), dtype=np.uint64)
+    off = 0
+    for i, sz in enumerate(sizes.tolist()):
+        doc = datagen.sexp_doc(int(sz), seed=args.seed + 1000 + i)
+        d_in.upload(doc, offset=off)
+        offsets[i] = off
+        off += int(sz)
+    lengths = sizes.astype(np.uint64)
+    ctx = Context(bundle)
+    # size the outputs with one call (it reports the node count when the capacity is too small)
+    rc, need, res = ctx.parse_batch_device(d_in.ptr, offsets, lengths, 0, 0, 0, 0)
+    cap = int(need)
+    outs = [DeviceBuffer(4 * cap, lib), DeviceBuffer(4 * cap, lib), DeviceBuffer(8 * cap, lib)]
+
+    def batch_step():
+        return ctx.parse_batch_device(d_in.ptr, offsets, lengths, outs[0].ptr, outs[1].ptr,
+                                      outs[2].ptr, cap)
+
+    rc, got, res = batch_step()
+    if rc != 1:
+        raise SystemExit(f"batched call failed (rc={rc})")
+    accepted = int((res["status"] == 1).sum())
+    ctx.profile(True)
+    lib.alpacc_stream_sync(None)
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        batch_step()
+    lib.alpacc_stream_sync(None)
+    t_batch = (time.perf_counter() - t0) / args.batch_steps
+    prof = ctx.read_profile()
+    ctx.profile(False)
+    # plain: one alpacc_parse call per document, same documents and output buffers
+    log("plain per-document calls")
+    t0 = time.perf_counter()
+    plain_nodes = 0
+    plain_acc = 0
+    for i in range(len(sizes)):
+        rc_i, cnt = ctx.parse_device(d_in.ptr + int(offsets[i]), int(lengths[i]), outs[0].ptr,
+                                     outs[1].ptr, outs[2].ptr, cap)
+        plain_acc += rc_i == 1
+        plain_nodes += cnt if rc_i == 1 else 0
+    lib.alpacc_stream_sync(None)
+    t_plain = time.perf_counter() - t0
+    if plain_acc != accepted or plain_nodes != int(res["count"].sum()):
+        raise SystemExit("batched and per-document results disagree")
+    out = {"workload": f"sexp lex+parse, {len(sizes)} documents, {total >> 20} MiB, batched "
+                       "(alpacc_parse_batch) vs one alpacc_parse per document",
+           "docs": len(sizes), "bytes": total, "accepted": accepted, "nodes": int(got),
+           "batched_ms": round(t_batch * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
+           "batched_docs_per_s": round(len(sizes) / t_batch, 1),
+           "plain_docs_per_s": round(len(sizes) / t_plain, 1),
+           "batched_GBps": round(total / t_batch / 1e9, 3),
+           "vs_plain": round(t_plain / t_batch, 2),
+           "batched_stage_ms": {k: round(v[0] / v[1], 4) for k, v in prof.items() if v[1]}}
+    ctx.close()
+    for o in outs:
+        o.free()
+    d_in.free()
+    return out
+
+
+def serving_bench(args):
+    """Poisson stream of json lex+parse requests (log-uniform 256 B..64 MiB, 10 % invalid) served
+    by continuous batching, and by one alpacc_parse call per request, at the same arrivals."""
+    import numpy as np
+    from alpacc_amd import datagen, serving
+    from alpacc_amd.gen.tables import generate
+    from alpacc_amd.runtime import Context
+    bundle = generate(os.path.join(ROOT, "alpacc_amd", "grammars", "json.alp"))
+    sizes = serving.request_sizes(args.serve_requests, seed=args.seed)
+    rng = np.random.Generator(np.random.PCG64(args.seed + 7))
+    log(f"generating {len(sizes)} requests, {int(sizes.sum()) >> 20} MiB")
+    datas, invalid = [], 0
+    for i, n in enumerate(sizes.tolist()):
+        d = datagen.json_doc(max(int(n), 2), seed=args.seed + 20_000 + i)
+        if rng.random() < 0.1:
+            d = datagen.flip_bytes(d, seed=i, block=min(1 << 20, d.shape[0]))
+            invalid += 1
+        datas.append(d)
+    arrivals = serving.poisson_arrivals(len(datas), args.serve_rate, seed=args.seed)
+    # warm both paths (scratch arenas grow to the largest request once)
+    srv = serving.Server(bundle, workers=args.serve_workers)
+    serving.serve_poisson(srv, datas[:8], np.zeros(8))
+    log(f"serving at {args.serve_rate:g} requests/s")
+    reqs = serving.serve_poisson(srv, datas, arrivals)
+    srv.close()
+    batched = serving.latency_stats(reqs)
+    batched["launches"] = len(srv.batch_sizes)
+    batched["mean_batch"] = round(float(np.mean(srv.batch_sizes)), 2)
+    ctx = Context(bundle)
+    ctx.parse_int(max(datas, key=len))
+    log("plain loop")
+    plain_reqs = [serving.Request(i, np.ascontiguousarray(d)) for i, d in enumerate(datas)]
+    serving.serve_plain(ctx, plain_reqs, arrivals)
+    ctx.close()
+    plain = serving.latency_stats(plain_reqs)
+    same = all((a.status == 1) == (b.status == 1) and
+               (a.tree is None or all(np.array_equal(x, y) for x, y in zip(a.tree, b.tree)))
+               for a, b in zip(reqs, plain_reqs))
+    return {"workload": f"json lex+parse requests, Poisson arrivals at {args.serve_rate:g}/s, "
+                        f"sizes log-uniform 256 B..64 MiB, {invalid} of {len(datas)} invalid",
+            "offered_GBps": round(float(sizes.sum()) / float(arrivals[-1]) / 1e9, 3),
+            "continuous_batching": batched, "one_request_per_launch": plain,
+            "results_identical": bool(same), "workers": args.serve_workers,
+            "hbm_budget_GB": 200}
 
 
 def main():
@@ -125,19 +254,34 @@ def main():
     ms_step = (t1 - t0) * 1000.0 / args.steps
     value = n / (ms_step / 1000.0) / 1e9
     stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in prof.items()}
-    # per-stage algorithmic bytes (compulsory traffic only):
-    #   lex_reduce: n (reads the document), lex_emit: n + 12 T (reads it again, writes the tokens),
-    #   lex_scan: 0 (tile aggregates only), parse: 16 N (writes the tree; tokens are intermediates)
-    alg = {"lex_reduce": n, "lex_scan": 0, "lex_emit": n + 12 * T}
+    # Algorithmic bytes per launch of every single-kernel stage: what the kernel must read and
+    # write at the least (per unit: byte n, token T, key position P = T + 2, bracket B, node N).
+    #   lex_reduce    : n                       reads the document
+    #   lex_emit      : n + 12 T                reads it again, writes (kind, start, end) u32
+    #   parse_expand  : 2 P + 8 T + 6 B + 16 N  pair entries in; token spans gathered; depth + symbol
+    #                                           per bracket; s, node word and span per node out
+    #   bracket_match : 6 B                     depth + symbol per bracket
+    #   parse_parents : 8 N                     s in, parent out
+    cnt = ctx.last_counts() if args.workload == "parse" else {}
+    P, B, N_ = cnt.get("positions", 0), cnt.get("brackets", 0), cnt.get("productions", 0)
+    alg = {"lex_reduce": n, "lex_emit": n + 12 * T}
     if args.workload == "parse":
-        alg["parse"] = 16 * count
+        alg.update({"parse_expand": 2 * P + 8 * T + 6 * B + 16 * N_, "bracket_match": 6 * B,
+                    "parse_parents": 8 * N_})
     dom = max((k for k in alg if stage_ms.get(k, 0) > 0), key=lambda k: stage_ms[k])
     dom_gbs = alg[dom] / (stage_ms[dom] / 1000.0) / 1e9 if alg[dom] else 0.0
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbs, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
             "traffic": None, "avg_ms": round(stage_ms[dom], 4), "algorithmic_bytes": int(alg[dom])}
     lex_ms = sum(stage_ms.get(k, 0.0) for k in ("lex_reduce", "lex_scan", "lex_emit"))
-    extra = {"tokens": int(T), "stage_ms": {k: round(v, 4) for k, v in stage_ms.items() if v},
+    dev_ms = lex_ms + stage_ms.get("parse", 0.0)
+    pipe_bytes = n + 16 * count if args.workload == "parse" else n + 12 * T
+    pipeline = {"algorithmic_bytes": int(pipe_bytes), "device_ms": round(dev_ms, 4),
+                "achieved": round(pipe_bytes / (dev_ms / 1000.0) / 1e9, 1),
+                "frac": round(pipe_bytes / (dev_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+                "counts": cnt}
+    extra = {"tokens": int(T), "pipeline_roofline": pipeline,
+             "stage_ms": {k: round(v, 4) for k, v in stage_ms.items() if v},
              "lexer_pipeline_GBps": round((n + 12 * T) / (lex_ms / 1000.0) / 1e9, 1) if lex_ms else None,
              "tokens_per_s": round(T / (ms_step / 1000.0), 1)}
     if args.workload == "parse":
@@ -164,6 +308,15 @@ def main():
             cpu = cpu_baseline(bundle, args.workload, args.cpu_sample_mib, args.seed)
         except Exception as exc:  # the baseline is reported, never required
             log(f"cpu baseline unavailable: {exc}")
+    if not args.no_batch or not args.no_serve:
+        for o in outs:
+            o.free()
+        d_in.free()
+        ctx.close()
+    if not args.no_batch:
+        extra["batched"] = batched(args, lib)
+    if not args.no_serve:
+        extra["serving"] = serving_bench(args)
     metric = ("json_lex_parse_input_GBps" if args.workload == "parse" else "json_lex_input_GBps")
     line = {
         "metric": metric, "value": round(value, 3), "unit": "GB/s", "n_gpus": 1,

@@ -121,11 +121,48 @@ int alpacc_parse_chunked(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint6
                          uint32_t *d_parent, uint32_t *d_node, uint64_t *d_span, uint64_t cap,
                          uint64_t *num_nodes, void *stream);
 
+/* Per-document result of a batched call: status ALPACC_ACCEPT / ALPACC_REJECT, the document's
+ * first output slot (token slot or tree node) and its count (0 when rejected). */
+typedef struct alpacc_doc_result {
+  int32_t status;
+  uint32_t reserved;
+  uint64_t offset;
+  uint64_t count;
+} alpacc_doc_result;
+
+/* Batched lex_int over num_docs documents of one device buffer (document d = bytes
+ * [offsets[d], offsets[d] + lengths[d]) of d_in; host arrays), each lexed on its own with the
+ * unchunked semantics. Document d's tokens are written from slot sum_{d' < d}(lengths[d'] + 1) of
+ * d_kinds/d_starts/d_ends (u32, positions relative to the document); cap >= sum(lengths + 1).
+ * results[d] (host) receives status, first slot and token count. Returns ALPACC_ACCEPT when the
+ * batch ran (per-document verdicts are in results), < 0 on error. There is no counterpart entry in
+ * the reference: it replaces one lex_int call per document. */
+int alpacc_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
+                     const uint64_t *lengths, uint32_t num_docs, uint32_t *d_kinds,
+                     uint32_t *d_starts, uint32_t *d_ends, uint64_t cap,
+                     alpacc_doc_result *results, void *stream);
+
+/* Batched parse_int (lexer + parser) over num_docs documents, as alpacc_lex_batch. The trees are
+ * written one after the other to d_parent/d_node/d_span (u32, span u32x2); results[d].offset is
+ * the document's first node, parents are relative to it (each document's first node is its own
+ * parent) and spans relative to the document. *total_nodes receives the nodes written (also the
+ * size needed when the call returns ALPACC_ERR_CAPACITY). Replaces one parse_int call per
+ * document of the reference's generated entry (Generator.hs:35-44). */
+int alpacc_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
+                       const uint64_t *lengths, uint32_t num_docs, uint32_t *d_parent,
+                       uint32_t *d_node, uint32_t *d_span, uint64_t cap, uint64_t *total_nodes,
+                       alpacc_doc_result *results, void *stream);
+
 /* Number of kernels the last call on `ctx` launched. */
 int alpacc_last_launch_count(alpacc_ctx *ctx);
 
+/* Element counts of the last parser call on `ctx`: counts[0] tokens, [1] key positions (T + 2;
+ * tile-padded in batched calls), [2] brackets, [3] productions (tree nodes). */
+int alpacc_last_counts(alpacc_ctx *ctx, uint64_t *counts);
+
 /* Device-time profiling: when enabled, every call records HIP events on its stream around each
- * stage (lex_reduce, lex_scan, lex_emit, parse). alpacc_profile_read fills up to max_stages
+ * stage (lex_reduce, lex_scan, lex_emit, parse, and inside parse: parse_keys, parse_expand,
+ * bracket_match, parse_parents). alpacc_profile_read fills up to max_stages
  * names, accumulated milliseconds and call counts since the last read, resets them, and returns
  * the number of stages filled. */
 int alpacc_profile_enable(alpacc_ctx *ctx, int on);
@@ -139,6 +176,8 @@ int alpacc_device_count(void);
 int alpacc_set_device(int device);
 void *alpacc_device_alloc(uint64_t bytes);
 void alpacc_device_free(void *ptr);
+void *alpacc_host_alloc(uint64_t bytes); /* pinned host memory */
+void alpacc_host_free(void *ptr);
 int alpacc_copy_to_device(void *dst, const void *src, uint64_t bytes, void *stream);
 int alpacc_copy_to_host(void *dst, const void *src, uint64_t bytes, void *stream);
 int alpacc_device_memset(void *dst, int value, uint64_t bytes, void *stream);

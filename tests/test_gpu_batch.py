@@ -1,0 +1,114 @@
+"""GPU parity of the batched entries (alpacc_lex_batch / alpacc_parse_batch): every document of a
+batch must get exactly what lex_int / parse_int give it alone (the C restatement of lexer.fut +
+parser.fut as the oracle), including empty, tiny, whitespace-only and invalid documents, documents
+at tile boundaries, and many tiny documents sharing tiles and 64-element chunks."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GRAMMARS
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctxs():
+    from alpacc_amd.gen.tables import generate
+    from alpacc_amd.runtime import Context
+    out = {}
+    for g in ("json", "arithmetic", "sexp"):
+        b = generate(os.path.join(GRAMMARS, f"{g}.alp"))
+        out[g] = (b, Context(b))
+    yield out
+    for _, c in out.values():
+        c.close()
+
+
+GEN = {"json": "json_doc", "arithmetic": "arith_doc", "sexp": "sexp_doc"}
+
+
+def _docs(g, seed):
+    from alpacc_amd import datagen
+    gen = getattr(datagen, GEN[g])
+    rng = np.random.default_rng(seed)
+    docs = [b"", b" ", b"\n\n", b"(", b"1", b"a"]
+    for n in (2, 3, 5, 100, 2047, 2048, 2049, 4095, 4096, 4097, 10_000, 65_537, 100_000):
+        d = gen(n, seed=seed + n)
+        docs.append(d.tobytes())
+        docs.append(datagen.flip_bytes(d, seed=seed + n, block=max(16, n // 3)).tobytes())
+    rng.shuffle(docs)
+    return docs
+
+
+def _lex_rows(r):
+    return None if r is None else [tuple(int(v) for v in t) for t in zip(*r)]
+
+
+def _tree(r):
+    if r is None:
+        return None
+    par, node, span = r
+    span = np.asarray(span).reshape(-1)
+    return (np.asarray(par).tolist(), np.asarray(node).tolist(), span.tolist())
+
+
+@pytest.mark.parametrize("g", ["json", "arithmetic", "sexp"])
+def test_lex_batch_matches_single(ctxs, g):
+    from oracle.c import coracle
+    b, ctx = ctxs[g]
+    orc = coracle.Oracle(b)
+    docs = _docs(g, 11)
+    print(f"lex batch {g}: {len(docs)} docs", flush=True)
+    got = ctx.lex_int_batch(docs)
+    for i, d in enumerate(docs):
+        assert _lex_rows(got[i]) == _lex_rows(orc.lex(d)), (g, i, len(d))
+
+
+@pytest.mark.parametrize("g", ["json", "arithmetic", "sexp"])
+def test_parse_batch_matches_single(ctxs, g):
+    from oracle.c import coracle
+    b, ctx = ctxs[g]
+    orc = coracle.Oracle(b)
+    docs = _docs(g, 12)
+    print(f"parse batch {g}: {len(docs)} docs", flush=True)
+    got = ctx.parse_int_batch(docs)
+    for i, d in enumerate(docs):
+        assert _tree(got[i]) == _tree(orc.parse(d)), (g, i, len(d))
+
+
+@pytest.mark.parametrize("g", ["sexp", "arithmetic"])
+def test_many_tiny_documents(ctxs, g):
+    """Thousands of documents of 1-300 bytes: several documents per wave tile of the lexer, per
+    position tile of the parser and per 64-element chunk of the searches."""
+    from alpacc_amd import datagen
+    from oracle.c import coracle
+    b, ctx = ctxs[g]
+    orc = coracle.Oracle(b)
+    gen = getattr(datagen, GEN[g])
+    rng = np.random.default_rng(5)
+    docs = []
+    for i in range(3000):
+        n = int(rng.integers(2, 300))
+        d = gen(n, seed=1000 + i)
+        if i % 7 == 3:
+            d = datagen.flip_bytes(d, seed=i, block=max(8, n // 2))
+        docs.append(d.tobytes())
+    print(f"tiny docs {g}: {len(docs)}", flush=True)
+    lex = ctx.lex_int_batch(docs)
+    par = ctx.parse_int_batch(docs)
+    for i, d in enumerate(docs):
+        assert _lex_rows(lex[i]) == _lex_rows(orc.lex(d)), (g, i)
+        assert _tree(par[i]) == _tree(orc.parse(d)), (g, i)
+
+
+def test_batch_capacity_retry(ctxs):
+    """A too-small output capacity reports the size needed; the retried call succeeds."""
+    from alpacc_amd import datagen
+    from oracle.c import coracle
+    b, ctx = ctxs["json"]
+    docs = [datagen.json_doc(5000, seed=s).tobytes() for s in range(3)]
+    got = ctx.parse_int_batch(docs, cap=16)
+    orc = coracle.Oracle(b)
+    for i, d in enumerate(docs):
+        assert _tree(got[i]) == _tree(orc.parse(d))
