@@ -2194,7 +2194,7 @@ int elem_bufs(alpacc_ctx *ctx, uint64_t B, uint64_t N, ElemBufs &e, hipStream_t 
 template <typename S>
 int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, const S *ends,
                      uint64_t T, uint32_t *parent, uint32_t *node, S *span, uint64_t cap,
-                     uint64_t *num_nodes, hipStream_t st) {
+                     uint64_t *num_nodes, hipStream_t st, bool productions_only = false) {
   if (!ctx->has_parser) {
     ctx->err = "bundle has no parser";
     return ALPACC_ERR_NO_PARSER;
@@ -2240,7 +2240,7 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, co
   // depths (parser.fut:153-166): a prefix below zero, or a non-zero end
   if (tot.nb > 0 && (tot.mind < 0 || tot.dd != 0)) return ALPACC_REJECT;
   const uint64_t B = (uint32_t)tot.nb, N = (uint32_t)tot.np;
-  a.subst = (uint64_t)(uint32_t)tot.nt == T ? 1 : 0;
+  a.subst = (!productions_only && (uint64_t)(uint32_t)tot.nt == T) ? 1 : 0;
   ctx->last_counts[0] = T;
   ctx->last_counts[1] = P;
   ctx->last_counts[2] = B;
@@ -2873,6 +2873,332 @@ int alpacc_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *off
   return rc;
 }
 
+}  // extern "C"
+
+namespace {
+
+// test.fut wire format: big-endian u64 (encode_u64, test.fut:9-18), one-byte booleans.
+void put_u64(std::vector<uint8_t> &o, uint64_t v) {
+  for (int k = 7; k >= 0; --k) o.push_back((uint8_t)(v >> (8 * k)));
+}
+
+bool get_u64(const uint8_t *b, uint64_t size, uint64_t &pos, uint64_t &v) {
+  if (pos + 8 > size) return false;
+  v = 0;
+  for (int k = 0; k < 8; ++k) v = (v << 8) | b[pos + k];
+  pos += 8;
+  return true;
+}
+
+struct DevArr {
+  void *p = nullptr;
+  size_t n = 0;
+  void *get(size_t bytes) {
+    if (n < bytes) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      n = 0;
+      if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) return nullptr;
+      n = std::max<size_t>(bytes, 256);
+    }
+    return p;
+  }
+  ~DevArr() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// Lexer+parser or lexer-only artifact: every input through parse_int / lex_int of the generated
+// entries, i.e. with the reference's 16 MiB chunk (Generator.hs:24-76). Inputs up to one chunk go
+// through the batched entry (one chunk: both modes agree), longer ones through the chunked call.
+int test_bytes(alpacc_ctx *ctx, const std::vector<std::pair<uint64_t, uint64_t>> &ins,
+               const uint8_t *blob, std::vector<uint8_t> &out, hipStream_t st) {
+  const bool parse = ctx->has_parser;
+  const uint64_t C = ALPACC_REFERENCE_CHUNK;
+  const uint32_t D = (uint32_t)ins.size();
+  std::vector<int32_t> status(D, 0);
+  std::vector<std::vector<uint8_t>> rec(D);
+  // documents up to one chunk: one batched call
+  std::vector<uint32_t> small;
+  std::vector<uint64_t> offs, lens;
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < D; ++i)
+    if (ins[i].second <= C) {
+      small.push_back(i);
+      offs.push_back(total);
+      lens.push_back(ins[i].second);
+      total += ins[i].second;
+    }
+  DevArr din, d0, d1, d2;
+  uint8_t *dev = (uint8_t *)din.get(total + 16);
+  if (!dev) return ALPACC_ERR_HIP;
+  for (size_t j = 0; j < small.size(); ++j)
+    if (lens[j] && hipMemcpyAsync(dev + offs[j], blob + ins[small[j]].first, lens[j],
+                                  hipMemcpyHostToDevice, st) != hipSuccess)
+      return ALPACC_ERR_HIP;
+  if (!small.empty()) {
+    std::vector<alpacc_doc_result> res(small.size());
+    if (parse) {
+      uint64_t cap = total + 16, nodes = 0;
+      int rc;
+      for (;;) {
+        void *p0 = d0.get(cap * 4), *p1 = d1.get(cap * 4), *p2 = d2.get(cap * 8);
+        if (!p0 || !p1 || !p2) return ALPACC_ERR_HIP;
+        rc = run_parse_batch(ctx, dev, offs.data(), lens.data(), (uint32_t)small.size(),
+                             (uint32_t *)p0, (uint32_t *)p1, (uint32_t *)p2, cap, &nodes,
+                             res.data(), st);
+        if (rc == ALPACC_ERR_CAPACITY) {
+          cap = nodes + 16;
+          continue;
+        }
+        break;
+      }
+      if (rc < 0) return rc;
+      std::vector<uint32_t> par(nodes), nd(nodes), sp(2 * nodes);
+      if (nodes &&
+          (hipMemcpyAsync(par.data(), d0.p, nodes * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+           hipMemcpyAsync(nd.data(), d1.p, nodes * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+           hipMemcpyAsync(sp.data(), d2.p, nodes * 8, hipMemcpyDeviceToHost, st) != hipSuccess))
+        return ALPACC_ERR_HIP;
+      if (hipStreamSynchronize(st) != hipSuccess) return ALPACC_ERR_HIP;
+      for (size_t j = 0; j < small.size(); ++j) {
+        std::vector<uint8_t> &o = rec[small[j]];
+        if (res[j].status != ALPACC_ACCEPT) continue;
+        status[small[j]] = 1;
+        put_u64(o, res[j].count);
+        for (uint64_t k = res[j].offset; k < res[j].offset + res[j].count; ++k) {
+          const bool term = (nd[k] & ALPACC_NODE_TERMINAL) != 0;
+          o.push_back(term ? 1 : 0);
+          put_u64(o, par[k]);
+          put_u64(o, nd[k] & ~ALPACC_NODE_TERMINAL);
+          put_u64(o, term ? sp[2 * k] : 0);
+          put_u64(o, term ? sp[2 * k + 1] : 0);
+        }
+      }
+    } else {
+      const uint64_t cap = total + small.size() + 1;
+      void *p0 = d0.get(cap * 4), *p1 = d1.get(cap * 4), *p2 = d2.get(cap * 4);
+      if (!p0 || !p1 || !p2) return ALPACC_ERR_HIP;
+      std::vector<uint32_t> doc_res_unused;
+      uint32_t *doc_res = (uint32_t *)ctx->buf(B_DOC_RES, (2ull * small.size() + 2) * 4);
+      alpacc_doc_result *d_results =
+          (alpacc_doc_result *)ctx->buf(B_RESULTS, (small.size() + 1) * sizeof(alpacc_doc_result));
+      if (!doc_res || !d_results) return ALPACC_ERR_HIP;
+      LexDoc *ldocs = nullptr;
+      int rc = run_lex_batch(ctx, dev, offs.data(), lens.data(), (uint32_t)small.size(),
+                             (uint32_t *)p0, (uint32_t *)p1, (uint32_t *)p2, doc_res, &ldocs, st);
+      if (rc < 0) return rc;
+      lb_finish<<<((uint32_t)small.size() + 255) / 256, 256, 0, st>>>(ldocs, doc_res,
+                                                                       (uint32_t)small.size(),
+                                                                       d_results);
+      std::vector<uint32_t> k(cap), sa(cap), ea(cap);
+      if (hipMemcpyAsync(res.data(), d_results, small.size() * sizeof(alpacc_doc_result),
+                         hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipMemcpyAsync(k.data(), p0, cap * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipMemcpyAsync(sa.data(), p1, cap * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipMemcpyAsync(ea.data(), p2, cap * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return ALPACC_ERR_HIP;
+      for (size_t j = 0; j < small.size(); ++j) {
+        std::vector<uint8_t> &o = rec[small[j]];
+        if (res[j].status != ALPACC_ACCEPT) continue;
+        status[small[j]] = 1;
+        put_u64(o, res[j].count);
+        for (uint64_t t = res[j].offset; t < res[j].offset + res[j].count; ++t) {
+          put_u64(o, k[t]);
+          put_u64(o, sa[t]);
+          put_u64(o, ea[t]);
+        }
+      }
+    }
+  }
+  // documents longer than one chunk: the chunked calls
+  for (uint32_t i = 0; i < D; ++i) {
+    const uint64_t n = ins[i].second;
+    if (n <= C) continue;
+    uint8_t *d = (uint8_t *)din.get(n + 16);
+    if (!d || hipMemcpyAsync(d, blob + ins[i].first, n, hipMemcpyHostToDevice, st) != hipSuccess)
+      return ALPACC_ERR_HIP;
+    std::vector<uint8_t> &o = rec[i];
+    if (parse) {
+      const uint64_t tcap = n + (n + C - 1) / C + 1;
+      uint32_t *kinds = (uint32_t *)ctx->buf(B_TOK_KIND, tcap * 4);
+      uint64_t *s64 = (uint64_t *)ctx->buf(B_TOK_START64, tcap * 8);
+      uint64_t *e64 = (uint64_t *)ctx->buf(B_TOK_END64, tcap * 8);
+      if (!kinds || !s64 || !e64) return ALPACC_ERR_HIP;
+      uint64_t T = 0, N = 0;
+      int rc = run_lex(ctx, d, n, kinds, nullptr, nullptr, &T, st, C, s64, e64);
+      if (rc < 0) return rc;
+      if (rc != ALPACC_ACCEPT) continue;
+      uint64_t cap = 2 * n + 16;
+      for (;;) {
+        void *p0 = d0.get(cap * 4), *p1 = d1.get(cap * 4), *p2 = d2.get(cap * 16);
+        if (!p0 || !p1 || !p2) return ALPACC_ERR_HIP;
+        rc = run_parse_tokens<uint64_t>(ctx, kinds, s64, e64, T, (uint32_t *)p0, (uint32_t *)p1,
+                                        (uint64_t *)p2, cap, &N, st);
+        if (rc == ALPACC_ERR_CAPACITY) {
+          cap = N + 16;
+          continue;
+        }
+        break;
+      }
+      if (rc < 0) return rc;
+      if (rc != ALPACC_ACCEPT) continue;
+      std::vector<uint32_t> par(N), nd(N);
+      std::vector<uint64_t> sp(2 * N);
+      if (hipMemcpyAsync(par.data(), d0.p, N * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipMemcpyAsync(nd.data(), d1.p, N * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipMemcpyAsync(sp.data(), d2.p, N * 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return ALPACC_ERR_HIP;
+      status[i] = 1;
+      put_u64(o, N);
+      for (uint64_t k = 0; k < N; ++k) {
+        const bool term = (nd[k] & ALPACC_NODE_TERMINAL) != 0;
+        o.push_back(term ? 1 : 0);
+        put_u64(o, par[k]);
+        put_u64(o, nd[k] & ~ALPACC_NODE_TERMINAL);
+        put_u64(o, term ? sp[2 * k] : 0);
+        put_u64(o, term ? sp[2 * k + 1] : 0);
+      }
+    } else {
+      const uint64_t cap = n + (n + C - 1) / C + 1;
+      void *p0 = d0.get(cap * 4), *p1 = d1.get(cap * 8), *p2 = d2.get(cap * 8);
+      if (!p0 || !p1 || !p2) return ALPACC_ERR_HIP;
+      uint64_t T = 0;
+      int rc = run_lex(ctx, d, n, (uint32_t *)p0, nullptr, nullptr, &T, st, C, (uint64_t *)p1,
+                       (uint64_t *)p2);
+      if (rc < 0) return rc;
+      if (rc != ALPACC_ACCEPT) continue;
+      std::vector<uint32_t> k(T);
+      std::vector<uint64_t> sa(T), ea(T);
+      if (T && (hipMemcpyAsync(k.data(), p0, T * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(sa.data(), p1, T * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(ea.data(), p2, T * 8, hipMemcpyDeviceToHost, st) != hipSuccess))
+        return ALPACC_ERR_HIP;
+      if (hipStreamSynchronize(st) != hipSuccess) return ALPACC_ERR_HIP;
+      status[i] = 1;
+      put_u64(o, T);
+      for (uint64_t t = 0; t < T; ++t) {
+        put_u64(o, k[t]);
+        put_u64(o, sa[t]);
+        put_u64(o, ea[t]);
+      }
+    }
+  }
+  for (uint32_t i = 0; i < D; ++i) {
+    out.push_back(status[i] ? 1 : 0);
+    if (status[i]) out.insert(out.end(), rec[i].begin(), rec[i].end());
+  }
+  return 0;
+}
+
+// Parser-only artifact (parser_test, test.fut:60-98): every input is an array of u64 terminals,
+// the output its production sequence (pre_productions_int).
+int test_terminals(alpacc_ctx *ctx, const uint8_t *blob, uint64_t size, uint64_t pos,
+                   uint64_t count, std::vector<uint8_t> &out, hipStream_t st) {
+  DevArr dk, ds, d0, d1, d2;
+  for (uint64_t i = 0; i < count; ++i) {
+    uint64_t m = 0;
+    if (!get_u64(blob, size, pos, m) || m > (size - pos) / 8) return ALPACC_ERR_ARG;
+    std::vector<uint32_t> kinds(m);
+    bool fits = true;
+    for (uint64_t t = 0; t < m; ++t) {
+      uint64_t v = 0;
+      get_u64(blob, size, pos, v);
+      fits &= v < ctx->nT;
+      kinds[t] = (uint32_t)std::min<uint64_t>(v, 0xFFFFFFFFull);
+    }
+    if (!fits) {  // a terminal the table does not know: no key matches
+      out.push_back(0);
+      continue;
+    }
+    uint32_t *k = (uint32_t *)dk.get((m + 1) * 4);
+    uint32_t *zs = (uint32_t *)ds.get((m + 1) * 4);
+    if (!k || !zs) return ALPACC_ERR_HIP;
+    if (m && hipMemcpyAsync(k, kinds.data(), m * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+      return ALPACC_ERR_HIP;
+    uint64_t cap = 4 * m + 16, N = 0;
+    int rc;
+    for (;;) {
+      void *p0 = d0.get(cap * 4), *p1 = d1.get(cap * 4), *p2 = d2.get(cap * 8);
+      if (!p0 || !p1 || !p2) return ALPACC_ERR_HIP;
+      rc = run_parse_tokens<uint32_t>(ctx, k, zs, zs, m, (uint32_t *)p0, (uint32_t *)p1,
+                                      (uint32_t *)p2, cap, &N, st, true);
+      if (rc == ALPACC_ERR_CAPACITY) {
+        cap = N + 16;
+        continue;
+      }
+      break;
+    }
+    if (rc < 0) return rc;
+    if (rc != ALPACC_ACCEPT) {
+      out.push_back(0);
+      continue;
+    }
+    std::vector<uint32_t> nd(N);
+    if (N && hipMemcpyAsync(nd.data(), d1.p, N * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+      return ALPACC_ERR_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return ALPACC_ERR_HIP;
+    out.push_back(1);
+    put_u64(out, N);
+    for (uint64_t t = 0; t < N; ++t) put_u64(out, nd[t]);
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int alpacc_test(alpacc_ctx *ctx, const uint8_t *blob, uint64_t size, uint8_t **out,
+                uint64_t *out_size, void *stream) {
+  if (!ctx || !blob || !out || !out_size) return ALPACC_ERR_ARG;
+  *out = nullptr;
+  *out_size = 0;
+  ctx->launches = 0;
+  ctx->err.clear();
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = (hipStream_t)stream;
+  uint64_t pos = 0, count = 0;
+  if (!get_u64(blob, size, pos, count)) {
+    ctx->err = "test input shorter than its count";
+    return ALPACC_ERR_ARG;
+  }
+  std::vector<uint8_t> o;
+  put_u64(o, count);
+  int rc;
+  if (ctx->has_lexer) {
+    std::vector<std::pair<uint64_t, uint64_t>> ins;
+    for (uint64_t i = 0; i < count; ++i) {
+      uint64_t n = 0;
+      if (!get_u64(blob, size, pos, n) || n > size - pos) {
+        ctx->err = "test input truncated";
+        return ALPACC_ERR_ARG;
+      }
+      ins.emplace_back(pos, n);
+      pos += n;
+    }
+    rc = ins.empty() ? 0 : test_bytes(ctx, ins, blob, o, st);
+  } else if (ctx->has_parser) {
+    rc = test_terminals(ctx, blob, size, pos, count, o, st);
+  } else {
+    rc = ALPACC_ERR_ARG;
+  }
+  if (rc < 0) {
+    if (ctx->err.empty()) ctx->err = "test entry failed";
+    return rc;
+  }
+  uint8_t *buf = (uint8_t *)malloc(o.size() ? o.size() : 1);
+  if (!buf) return ALPACC_ERR_HIP;
+  memcpy(buf, o.data(), o.size());
+  *out = buf;
+  *out_size = o.size();
+  return ALPACC_ACCEPT;
+}
+
+void alpacc_free(void *ptr) { free(ptr); }
+
 int alpacc_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -2913,6 +3239,13 @@ int alpacc_copy_to_host(void *dst, const void *src, uint64_t bytes, void *stream
   hipStream_t st = (hipStream_t)stream;
   if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
   return hipStreamSynchronize(st) == hipSuccess ? 0 : -1;
+}
+
+int alpacc_copy_device(void *dst, const void *src, uint64_t bytes, void *stream) {
+  if (bytes == 0) return 0;
+  return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream) == hipSuccess
+             ? 0
+             : -1;
 }
 
 int alpacc_device_memset(void *dst, int value, uint64_t bytes, void *stream) {

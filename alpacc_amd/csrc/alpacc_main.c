@@ -7,9 +7,11 @@
  * header written by alpacc_amd.gen.emit_c; compile with
  *   -DALPACC_TABLES_HEADER='"json_tables.h"' -DALPACC_PREFIX=json
  *
- * usage: alpacc_<grammar> [lex|parse] [--names] < input
+ * usage: alpacc_<grammar> [lex|parse|test] [--names] < input
  *   lex   : "start end terminal" per token (the reference's WriteAscii format)
  *   parse : "parent T terminal start end" or "parent P production" per node, in preorder
+ *   test  : the artifact's `test` entry (futhark/test.fut): wire-format inputs on stdin,
+ *           wire-format outputs on stdout
  * --names prints terminal/production names instead of ids.
  */
 #include <stdint.h>
@@ -62,13 +64,14 @@ static int fail(const char *what, alpacc_ctx *ctx) {
 }
 
 int main(int argc, char **argv) {
-  int parse = 0, names = 0;
+  int parse = 0, names = 0, test = 0;
   for (int i = 1; i < argc; ++i) {
     if (!strcmp(argv[i], "parse")) parse = 1;
     else if (!strcmp(argv[i], "lex")) parse = 0;
+    else if (!strcmp(argv[i], "test")) test = 1;
     else if (!strcmp(argv[i], "--names")) names = 1;
     else {
-      fprintf(stderr, "usage: %s [lex|parse] [--names] < input\n", argv[0]);
+      fprintf(stderr, "usage: %s [lex|parse|test] [--names] < input\n", argv[0]);
       return 2;
     }
   }
@@ -80,6 +83,17 @@ int main(int argc, char **argv) {
   if (!ctx) {
     fprintf(stderr, "alpacc: %s\n", alpacc_create_error());
     return 2;
+  }
+  if (test) {
+    uint8_t *out = NULL;
+    uint64_t out_n = 0;
+    if (alpacc_test(ctx, in, n, &out, &out_n, NULL) < 0) return fail("test entry failed", ctx);
+    fwrite(out, 1, out_n, stdout);
+    fflush(stdout);
+    alpacc_free(out);
+    alpacc_destroy(ctx);
+    free(in);
+    return 0;
   }
   uint8_t *d_in = (uint8_t *)upload(in, n);
   if (!d_in) return fail("device upload failed", ctx);

@@ -60,7 +60,8 @@ EXPORTS = ("alpacc_create", "alpacc_create_error", "alpacc_destroy", "alpacc_las
            "alpacc_event_create", "alpacc_event_destroy", "alpacc_event_record",
            "alpacc_event_elapsed_ms", "alpacc_profile_enable", "alpacc_profile_read",
            "alpacc_lex_chunked", "alpacc_parse_chunked", "alpacc_lex_batch", "alpacc_parse_batch",
-           "alpacc_last_counts", "alpacc_host_alloc", "alpacc_host_free")
+           "alpacc_last_counts", "alpacc_host_alloc", "alpacc_host_free", "alpacc_copy_device",
+           "alpacc_test", "alpacc_free")
 
 # the chunk size of the reference's generated entries (Generator.hs:24-76: lexer.lex_int 16777216)
 REFERENCE_CHUNK = 16777216
@@ -92,6 +93,9 @@ def load_library():
     L.alpacc_lex_chunked.argtypes = [vp, vp, u64, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
     L.alpacc_parse_chunked.argtypes = [vp, vp, u64, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
     L.alpacc_last_counts.argtypes = [vp, vp]
+    L.alpacc_copy_device.argtypes = [vp, vp, u64, vp]
+    L.alpacc_test.argtypes = [vp, vp, u64, ctypes.POINTER(vp), ctypes.POINTER(u64), vp]
+    L.alpacc_free.argtypes = [vp]
     L.alpacc_host_alloc.argtypes = [u64]
     L.alpacc_host_alloc.restype = vp
     L.alpacc_host_free.argtypes = [vp]
@@ -447,6 +451,22 @@ class Context:
         nn = nn.value
         return (pb.download(np.uint32, nn, self.stream), nb.download(np.uint32, nn, self.stream),
                 spb.download(np.uint64, 2 * nn, self.stream).reshape(-1, 2))
+
+    def test(self, blob: bytes) -> bytes:
+        """The artifact's ``test`` entry (futhark/test.fut): wire-format inputs to wire-format
+        outputs (lexer, lexer+parser or parser-only form, from the bundle)."""
+        blob = bytes(blob)
+        buf = ctypes.create_string_buffer(blob, len(blob))
+        out = ctypes.c_void_p()
+        n = ctypes.c_uint64(0)
+        rc = self.lib.alpacc_test(self.handle, buf, len(blob), ctypes.byref(out), ctypes.byref(n),
+                                  self.stream)
+        if rc < 0:
+            raise self._err(rc)
+        try:
+            return ctypes.string_at(out, n.value)
+        finally:
+            self.lib.alpacc_free(out)
 
     # -- batched entries ---------------------------------------------------------------------
     def upload_batch(self, docs) -> tuple[DeviceBuffer, np.ndarray, np.ndarray]:

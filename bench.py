@@ -43,6 +43,9 @@ def parse_args():
     ap.add_argument("--batch-total-mib", type=int, default=2048)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving comparison")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run device-to-device copies of this size (the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=400.0, help="offered load, requests/s")
     ap.add_argument("--serve-workers", type=int, default=3)
@@ -205,6 +208,16 @@ def main():
     from alpacc_amd import datagen
     from alpacc_amd.gen.tables import generate
     from alpacc_amd.runtime import Context, DeviceBuffer, load_library
+
+    if args.copy_probe_mib:
+        lib = load_library()
+        nb = args.copy_probe_mib << 20
+        src, dst = DeviceBuffer(nb, lib), DeviceBuffer(nb, lib)
+        for _ in range(4):
+            lib.alpacc_copy_device(dst.ptr, src.ptr, nb, None)
+        lib.alpacc_stream_sync(None)
+        print(json.dumps({"copy_probe_bytes": nb, "copies": 4}), flush=True)
+        return
 
     mode = "both" if args.workload == "parse" else "lexer"
     bundle = generate(os.path.join(ROOT, "alpacc_amd", "grammars", "json.alp"), mode=mode)

@@ -153,6 +153,18 @@ int alpacc_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *off
                        uint32_t *d_node, uint32_t *d_span, uint64_t cap, uint64_t *total_nodes,
                        alpacc_doc_result *results, void *stream);
 
+/* The `test` entry of the generated artifact (futhark/test.fut:31-180, entries Generator.hs:19-92):
+ * blob = u64 count, then per input u64 length + bytes (or, for a parser-only bundle, u64 count of
+ * terminals + that many u64 terminals); all integers big-endian. The output, allocated here (free
+ * with alpacc_free), is u64 count, then per input one byte accepted and, when accepted:
+ *   lexer bundle     : u64 T, T x (u64 kind, start, end)               (lexer_test)
+ *   lexer + parser   : u64 N, N x (u8 tag, u64 parent, id, start, end)  (lexer_parser_test)
+ *   parser only      : u64 N, N x u64 production                        (parser_test)
+ * Lexing uses the reference entries' 16 MiB chunk. */
+int alpacc_test(alpacc_ctx *ctx, const uint8_t *blob, uint64_t size, uint8_t **out,
+                uint64_t *out_size, void *stream);
+void alpacc_free(void *ptr);
+
 /* Number of kernels the last call on `ctx` launched. */
 int alpacc_last_launch_count(alpacc_ctx *ctx);
 
@@ -180,6 +192,7 @@ void *alpacc_host_alloc(uint64_t bytes); /* pinned host memory */
 void alpacc_host_free(void *ptr);
 int alpacc_copy_to_device(void *dst, const void *src, uint64_t bytes, void *stream);
 int alpacc_copy_to_host(void *dst, const void *src, uint64_t bytes, void *stream);
+int alpacc_copy_device(void *dst, const void *src, uint64_t bytes, void *stream); /* async */
 int alpacc_device_memset(void *dst, int value, uint64_t bytes, void *stream);
 void *alpacc_stream_create(void);
 void alpacc_stream_destroy(void *stream);

@@ -5,6 +5,8 @@ All integers are big-endian u64 (``encode_u64``, test.fut:9-18); booleans are on
   * lexer outputs:     u64 count, then u8 ok [+ u64 T + T x (kind, start, end)] (test.fut:40-52)
   * lexer+parser:      u64 count, then u8 ok [+ u64 N + N x (u8 tag, parent, id, start, end)]
                        with tag 0 = production (spans 0) and 1 = terminal (test.fut:134-157)
+  * parser only:       inputs u64 count, per test u64 m + m x u64 terminal (test.fut:74-93);
+                       outputs u64 count, then u8 ok [+ u64 N + N x u64 production] (:68-72)
 """
 from __future__ import annotations
 
@@ -75,3 +77,22 @@ def decode_tree_outputs(blob: bytes) -> list:
             nodes.append((parent, bool(tag), ident, s, e))
         out.append(nodes)
     return out
+
+
+def encode_terminal_inputs(inputs) -> bytes:
+    out = [struct.pack(">Q", len(inputs))]
+    for ts in inputs:
+        out.append(struct.pack(">Q", len(ts)))
+        out.extend(struct.pack(">Q", int(t)) for t in ts)
+    return b"".join(out)
+
+
+def encode_production_outputs(results) -> bytes:
+    out = [struct.pack(">Q", len(results))]
+    for r in results:
+        if r is None:
+            out.append(b"\x00")
+            continue
+        out.append(b"\x01" + struct.pack(">Q", len(r)))
+        out.extend(struct.pack(">Q", int(p)) for p in r)
+    return b"".join(out)

@@ -75,3 +75,15 @@ def test_documents_and_rejections(bundles, g):
         assert (rc == 0) == (want is not None), (g, len(d))
         if want is not None:
             assert out == want
+
+
+def test_test_mode(bundles):
+    """`alpacc_<grammar> test` is the artifact's test entry: wire format in, wire format out."""
+    from oracle import sequential, wire
+    b = bundles["arithmetic"]
+    inputs = [b"12+(3*4)", b"(", b""]
+    exe = os.path.join(BIN, "alpacc_arithmetic")
+    r = subprocess.run([exe, "test"], input=wire.encode_inputs(inputs), capture_output=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr.decode()
+    assert r.stdout == wire.encode_tree_outputs([sequential.parse(b, s) for s in inputs])
