@@ -58,20 +58,55 @@ __device__ __forceinline__ void st_relaxed64(uint64_t *p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
+// Cross-lane steps through DPP row operations (register-to-register) instead of ds_bpermute.
+// update_dpp(old, src, ctrl, row_mask, bank_mask, bound_ctrl = false): a lane whose source lies
+// outside its 16-lane row, or whose row the mask disables, keeps `old` (the scan identity).
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114,
+              DPP_ROW_SHR8 = 0x118, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143,
+              DPP_WAVE_SHR1 = 0x138;
+
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int32_t dpp_i32(int32_t old, int32_t v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Inclusive scan in lane order: Hillis-Steele inside the rows (shr 1, 2, 4, 8), then row 0's
+// total into row 1 and row 2's into row 3 (bcast15), then rows 0-1 into rows 2-3 (bcast31).
+// `op(earlier, later)` may be non-commutative; `id` is its identity.
+template <class Op>
+__device__ __forceinline__ int32_t wave_incl_i32(int32_t v, int32_t id, Op op) {
+  v = op(dpp_i32<DPP_ROW_SHR1>(id, v), v);
+  v = op(dpp_i32<DPP_ROW_SHR2>(id, v), v);
+  v = op(dpp_i32<DPP_ROW_SHR4>(id, v), v);
+  v = op(dpp_i32<DPP_ROW_SHR8>(id, v), v);
+  v = op(dpp_i32<DPP_ROW_BCAST15, 0xA>(id, v), v);
+  v = op(dpp_i32<DPP_ROW_BCAST31, 0xC>(id, v), v);
   return v;
+}
+
+// the value of the lane before (lane 0 gets id)
+__device__ __forceinline__ int32_t wave_prev_i32(int32_t v, int32_t id) {
+  return dpp_i32<DPP_WAVE_SHR1>(id, v);
+}
+
+struct OpAdd {
+  __device__ int32_t operator()(int32_t a, int32_t b) const { return a + b; }
+};
+struct OpMax {
+  __device__ int32_t operator()(int32_t a, int32_t b) const { return max(a, b); }
+};
+struct OpMin {
+  __device__ int32_t operator()(int32_t a, int32_t b) const { return min(a, b); }
+};
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readlane(wave_incl_i32((int32_t)v, 0, OpAdd()), WAVE - 1);
 }
 __device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, WAVE));
-  return v;
+  return __builtin_amdgcn_readlane(wave_incl_i32(v, INT32_MIN, OpMax()), WAVE - 1);
 }
 __device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, WAVE));
-  return v;
+  return __builtin_amdgcn_readlane(wave_incl_i32(v, INT32_MAX, OpMin()), WAVE - 1);
 }
 
 struct LexArgs {
@@ -550,17 +585,8 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   const int32_t last = startm ? (int32_t)(base + 31 - __clz(startm)) : -1;
 
   // ---- scans: wave, count tile, across count tiles
-  uint32_t cincl = cnt;
-  int32_t lincl = last;
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) {
-    const uint32_t oc = __shfl_up(cincl, d, WAVE);
-    const int32_t ol = __shfl_up(lincl, d, WAVE);
-    if (lane >= d) {
-      cincl += oc;
-      lincl = max(lincl, ol);
-    }
-  }
+  const uint32_t cincl = (uint32_t)wave_incl_i32((int32_t)cnt, 0, OpAdd());
+  const int32_t lincl = wave_incl_i32(last, -1, OpMax());
   if (lane == WAVE - 1) {
     s_wcnt[w] = cincl;
     s_wlast[w] = lincl;
@@ -573,28 +599,32 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
     const uint32_t wc = lane < LEX_CT_WAVES ? s_wcnt[lane] : 0u;
     const int32_t wl = lane < LEX_CT_WAVES ? s_wlast[lane] : -1;
     const bool wr = lane < LEX_CT_WAVES ? s_wreset[lane] != 0 : false;
-    uint32_t wci = wc;
-    int32_t wli = wl;
-    bool wfi = wr;
-#pragma unroll
-    for (int d = 1; d < LEX_CT_WAVES; d <<= 1) {
-      const uint32_t oc = __shfl_up(wci, d, WAVE);
-      const int32_t ol = __shfl_up(wli, d, WAVE);
-      const bool of = __shfl_up((int)wfi, d, WAVE) != 0;
-      if (lane >= d && !wfi) {
-        wci += oc;
-        wli = max(wli, ol);
-        wfi = of;
-      }
+    // 16 lanes: one DPP row (shr 1, 2, 4, 8), segmented by the reset flags
+    int32_t wci = (int32_t)wc, wli = wl, wfi = wr ? 1 : 0;
+#define LEX_SEG_STEP(CTRL)                                  \
+    {                                                       \
+      const int32_t oc = dpp_i32<CTRL>(0, wci);             \
+      const int32_t ol = dpp_i32<CTRL>(-1, wli);            \
+      const int32_t of = dpp_i32<CTRL>(0, wfi);             \
+      if (!wfi) {                                           \
+        wci += oc;                                          \
+        wli = max(wli, ol);                                 \
+        wfi = of;                                           \
+      }                                                     \
     }
-    const uint32_t ctot = __shfl(wci, LEX_CT_WAVES - 1, WAVE);
-    const int32_t ltot = __shfl(wli, LEX_CT_WAVES - 1, WAVE);
-    const bool ftot = __shfl((int)wfi, LEX_CT_WAVES - 1, WAVE) != 0;
+    LEX_SEG_STEP(DPP_ROW_SHR1)
+    LEX_SEG_STEP(DPP_ROW_SHR2)
+    LEX_SEG_STEP(DPP_ROW_SHR4)
+    LEX_SEG_STEP(DPP_ROW_SHR8)
+#undef LEX_SEG_STEP
+    const uint32_t ctot = (uint32_t)__builtin_amdgcn_readlane(wci, LEX_CT_WAVES - 1);
+    const int32_t ltot = __builtin_amdgcn_readlane(wli, LEX_CT_WAVES - 1);
+    const bool ftot = __builtin_amdgcn_readlane(wfi, LEX_CT_WAVES - 1) != 0;
     // a count tile holding a document start already has its (post-reset) inclusive value
     if (lane == 0) st_relaxed64(p.st_count + ct, pack_count(ctot, ltot, ftot ? CT_INC : CT_AGG));
     uint32_t cpre = 0;
     int32_t lpre = -1;
-    const bool need_lb = ct > 0 && __shfl((int)wr, 0, WAVE) == 0;
+    const bool need_lb = ct > 0 && __builtin_amdgcn_readlane((int)wr, 0) == 0;
     if (need_lb) {
       lookback_count(p, ct, lane, cpre, lpre);
       if (lane == 0 && !ftot)
@@ -602,14 +632,9 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
     }
     // exclusive value of every wave: the inclusive value of the wave before (or nothing after a
     // reset), plus the look-back prefix when no reset precedes it in this count tile
-    uint32_t xc = __shfl_up(wci, 1, WAVE);
-    int32_t xl = __shfl_up(wli, 1, WAVE);
-    bool xf = __shfl_up((int)wfi, 1, WAVE) != 0;
-    if (lane == 0) {
-      xc = 0;
-      xl = -1;
-      xf = false;
-    }
+    uint32_t xc = (uint32_t)wave_prev_i32(wci, 0);
+    int32_t xl = wave_prev_i32(wli, -1);
+    const bool xf = wave_prev_i32(wfi, 0) != 0;
     if (wr) {
       xc = 0;
       xl = -1;
@@ -626,10 +651,9 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
 
   // ---- write tokens: stage in LDS by wave-local index, copy out coalesced, in rounds
   const uint32_t wout = (uint32_t)tv.tok_base + s_wcnt[w];  // slot of the wave's first token
-  const uint32_t wtotal = __shfl(cincl, WAVE - 1, WAVE);
+  const uint32_t wtotal = (uint32_t)__builtin_amdgcn_readlane((int32_t)cincl, WAVE - 1);
   const uint32_t lofs = cincl - cnt;                // wave-local index of this lane's first token
-  int32_t lexcl = __shfl_up(lincl, 1, WAVE);
-  if (lane == 0) lexcl = -1;
+  const int32_t lexcl = wave_prev_i32(lincl, -1);
   const int32_t start_in = max(max(s_wlast[w], lexcl), 0);
   uint32_t *stk = s_stage[w][0];
   uint32_t *sts = s_stage[w][1];
@@ -995,23 +1019,22 @@ __device__ __forceinline__ PTup ptup_of(const EntInfo &e) {
   return PTup{e.nb, e.np, e.dd, e.nb ? (int32_t)e.mind : INT32_MAX, e.ds, e.nt};
 }
 
-__device__ __forceinline__ PTup ptup_shfl_up(const PTup &v, int d) {
-  return PTup{__shfl_up(v.nb, d, WAVE), __shfl_up(v.np, d, WAVE), __shfl_up(v.dd, d, WAVE),
-              __shfl_up(v.mind, d, WAVE), __shfl_up(v.ds, d, WAVE), __shfl_up(v.nt, d, WAVE)};
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ PTup ptup_dpp(const PTup &v) {
+  return PTup{dpp_i32<CTRL, ROW_MASK>(0, v.nb),       dpp_i32<CTRL, ROW_MASK>(0, v.np),
+              dpp_i32<CTRL, ROW_MASK>(0, v.dd),       dpp_i32<CTRL, ROW_MASK>(INT32_MAX, v.mind),
+              dpp_i32<CTRL, ROW_MASK>(0, v.ds),       dpp_i32<CTRL, ROW_MASK>(0, v.nt)};
 }
 
-__device__ __forceinline__ PTup ptup_shfl(const PTup &v, int src) {
-  return PTup{__shfl(v.nb, src, WAVE), __shfl(v.np, src, WAVE), __shfl(v.dd, src, WAVE),
-              __shfl(v.mind, src, WAVE), __shfl(v.ds, src, WAVE), __shfl(v.nt, src, WAVE)};
-}
-
-// inclusive scan across the wavefront, in lane order
+// inclusive scan across the wavefront, in lane order (DPP rows, then row broadcasts)
 __device__ __forceinline__ PTup ptup_wave_incl(PTup v, int lane) {
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) {
-    const PTup o = ptup_shfl_up(v, d);
-    if (lane >= d) v = ptup_op(o, v);
-  }
+  (void)lane;
+  v = ptup_op(ptup_dpp<DPP_ROW_SHR1>(v), v);
+  v = ptup_op(ptup_dpp<DPP_ROW_SHR2>(v), v);
+  v = ptup_op(ptup_dpp<DPP_ROW_SHR4>(v), v);
+  v = ptup_op(ptup_dpp<DPP_ROW_SHR8>(v), v);
+  v = ptup_op(ptup_dpp<DPP_ROW_BCAST15, 0xA>(v), v);
+  v = ptup_op(ptup_dpp<DPP_ROW_BCAST31, 0xC>(v), v);
   return v;
 }
 
@@ -1028,8 +1051,7 @@ __device__ __forceinline__ PTup ptup_block_excl(PTup v, PTup *s_w, PTup &total) 
   for (int i = 0; i < w; ++i) pre = ptup_op(pre, s_w[i]);
   total = pre;
   for (int i = w; i < nw; ++i) total = ptup_op(total, s_w[i]);
-  PTup ex = ptup_shfl_up(incl, 1);
-  if (lane == 0) ex = ptup_id();
+  const PTup ex = ptup_dpp<DPP_WAVE_SHR1>(incl);  // lane 0: identity
   __syncthreads();
   return ptup_op(pre, ex);
 }
@@ -1252,8 +1274,9 @@ __device__ __forceinline__ SegTup seg_op(const SegTup &a, const SegTup &b) {
   return r;
 }
 
-__device__ __forceinline__ SegTup seg_shfl_up(const SegTup &v, int d) {
-  return SegTup{ptup_shfl_up(v.t, d), __shfl_up(v.f, d, WAVE)};
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ SegTup seg_dpp(const SegTup &v) {
+  return SegTup{ptup_dpp<CTRL, ROW_MASK>(v.t), dpp_i32<CTRL, ROW_MASK>(0, v.f)};
 }
 
 // Tile scan into `pre` (exclusive) and the grand total, in three kernels: block totals of 1024
@@ -1275,19 +1298,19 @@ __device__ __forceinline__ SegTup seg_block_excl(SegTup v, SegTup *s_w, SegTup &
   const int w = threadIdx.x / WAVE;
   const int nw = blockDim.x / WAVE;
   SegTup incl = v;
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) {
-    const SegTup o = seg_shfl_up(incl, d);
-    if (lane >= d) incl = seg_op(o, incl);
-  }
+  incl = seg_op(seg_dpp<DPP_ROW_SHR1>(incl), incl);
+  incl = seg_op(seg_dpp<DPP_ROW_SHR2>(incl), incl);
+  incl = seg_op(seg_dpp<DPP_ROW_SHR4>(incl), incl);
+  incl = seg_op(seg_dpp<DPP_ROW_SHR8>(incl), incl);
+  incl = seg_op(seg_dpp<DPP_ROW_BCAST15, 0xA>(incl), incl);
+  incl = seg_op(seg_dpp<DPP_ROW_BCAST31, 0xC>(incl), incl);
   if (lane == WAVE - 1) s_w[w] = incl;
   __syncthreads();
   SegTup pre{ptup_id(), 0};
   for (int i = 0; i < w; ++i) pre = seg_op(pre, s_w[i]);
   total = pre;
   for (int i = w; i < nw; ++i) total = seg_op(total, s_w[i]);
-  SegTup ex = seg_shfl_up(incl, 1);
-  if (lane == 0) ex = SegTup{ptup_id(), 0};
+  const SegTup ex = seg_dpp<DPP_WAVE_SHR1>(incl);  // lane 0: identity
   __syncthreads();
   return seg_op(pre, ex);
 }
@@ -1359,12 +1382,7 @@ __global__ void __launch_bounds__(PS_THREADS) pb_setup(PArgs a, const LexDoc *ld
     const uint32_t T = doc_res[2 * d + 1];
     acc += ok ? (uint32_t)((T + 2 + PK_TILE - 1) / PK_TILE) : 0u;
   }
-  uint32_t incl = acc;
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) {
-    const uint32_t o = __shfl_up(incl, d, WAVE);
-    if (lane >= d) incl += o;
-  }
+  const uint32_t incl = (uint32_t)wave_incl_i32((int32_t)acc, 0, OpAdd());
   if (lane == WAVE - 1) s_w[w] = incl;
   __syncthreads();
   uint32_t base = incl - acc;
@@ -1612,13 +1630,13 @@ __global__ void fill_i32(int32_t *p, uint64_t n, int32_t v) {
 // bracket at its previous smaller-or-equal depth. One wavefront per 64 brackets; a match inside
 // the wavefront's own chunk is read from a register.
 template <bool BATCH>
-__global__ void bracket_match(const uint16_t *br, uint64_t B, Levels lv, uint32_t *flags, PArgs a) {
+__device__ void bracket_match_run(const uint16_t *br, uint64_t B, const Levels &lv, uint32_t *flags,
+                                  const PArgs &a, uint64_t blk, uint64_t nblk) {
   const int lane = threadIdx.x & (WAVE - 1);
   const uint64_t nch = (B + WAVE - 1) / WAVE;
-  const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
+  const uint64_t wstride = nblk * (blockDim.x / WAVE);
   bool bad = false;
-  for (uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch;
-       c += wstride) {
+  for (uint64_t c = blk * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch; c += wstride) {
     const uint64_t i = c * WAVE + lane;
     const bool valid = i < B;
     const uint16_t b = valid ? br[i] : (uint16_t)0x8000u;
@@ -1656,12 +1674,12 @@ __global__ void bracket_match(const uint16_t *br, uint64_t B, Levels lv, uint32_
 // Batched: parents are relative to the document's first node, every document's first node is
 // its own parent.
 template <bool BATCH>
-__global__ void parse_parents(uint64_t N, Levels lv, uint32_t *parent, PArgs a) {
+__device__ void parents_run(uint64_t N, const Levels &lv, uint32_t *parent, const PArgs &a,
+                            uint64_t blk, uint64_t nblk) {
   const int lane = threadIdx.x & (WAVE - 1);
   const uint64_t nch = (N + WAVE - 1) / WAVE;
-  const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
-  for (uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch;
-       c += wstride) {
+  const uint64_t wstride = nblk * (blockDim.x / WAVE);
+  for (uint64_t c = blk * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch; c += wstride) {
     const uint64_t i = c * WAVE + lane;
     const bool valid = i < N;
     uint64_t base = 0;
@@ -1676,6 +1694,18 @@ __global__ void parse_parents(uint64_t N, Levels lv, uint32_t *parent, PArgs a) 
     const int64_t j = prev_le_wave(lv, c, lane, v, wv, valid && i > base);
     if (valid) parent[i] = i == base ? 0u : (uint32_t)((uint64_t)j - base);
   }
+}
+
+template <bool BATCH>
+__global__ void __launch_bounds__(256) bracket_match(const uint16_t *br, uint64_t B, Levels lv,
+                                                     uint32_t *flags, PArgs a) {
+  bracket_match_run<BATCH>(br, B, lv, flags, a, blockIdx.x, gridDim.x);
+}
+
+template <bool BATCH>
+__global__ void __launch_bounds__(256) parse_parents(uint64_t N, Levels lv, uint32_t *parent,
+                                                     PArgs a) {
+  parents_run<BATCH>(N, lv, parent, a, blockIdx.x, gridDim.x);
 }
 
 inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 16) {
@@ -1702,8 +1732,8 @@ enum Stage {
   S_PARSE_PARENTS,  // parse_parents
   NSTAGE
 };
-static const char *kStageNames[NSTAGE] = {"lex_reduce",   "lex_scan",      "lex_emit",
-                                          "parse",        "parse_keys",    "parse_expand",
+static const char *kStageNames[NSTAGE] = {"lex_reduce",    "lex_scan",     "lex_emit",
+                                          "parse",         "parse_keys",   "parse_expand",
                                           "bracket_match", "parse_parents"};
 
 struct alpacc_ctx {
@@ -1801,6 +1831,7 @@ enum BufId {
   B_BR,
   B_DEPTH,
   B_LEVELS,
+  B_LEVELS_S,
   B_SX,
   B_DL1,
   B_SL1,
@@ -1846,8 +1877,9 @@ T *upload(const T *h, size_t count, std::string &err) {
 
 // reduce-then-scan of n int2 values from `src` into `sink`; the grand total goes to *total
 // (copied to the host when total is not NULL).
+// buf_id: the scratch arena of the upper levels; hierarchies in use at the same time need their own
 int build_levels(alpacc_ctx *ctx, int32_t *base, int32_t *l1, uint64_t n, Levels &lv,
-                 hipStream_t st) {
+                 hipStream_t st, int buf_id) {
   lv.nlev = 1;
   lv.L[0] = base;
   lv.len[0] = n;
@@ -1861,7 +1893,7 @@ int build_levels(alpacc_ctx *ctx, int32_t *base, int32_t *l1, uint64_t n, Levels
     l = (l + 63) / 64;
     total += ((l + 3) & ~3ull) + 16;
   }
-  int32_t *area = (int32_t *)ctx->buf(B_LEVELS, (total + 16) * sizeof(int32_t));
+  int32_t *area = (int32_t *)ctx->buf(buf_id, (total + 16) * sizeof(int32_t));
   if (!area) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
@@ -2191,6 +2223,34 @@ int elem_bufs(alpacc_ctx *ctx, uint64_t B, uint64_t N, ElemBufs &e, hipStream_t 
   return 0;
 }
 
+// bracket matching over B brackets, then parents of N nodes (N = 0: no parents); the two min
+// hierarchies live in separate scratch arenas
+template <bool BATCH>
+int run_search(alpacc_ctx *ctx, const PArgs &a, const ElemBufs &e, uint64_t B, uint64_t N,
+               uint32_t *parent, uint32_t *flags, hipStream_t st) {
+  if (B == 0 && N == 0) return 0;
+  Levels lvb, lvs;
+  lvb.nlev = lvs.nlev = 0;
+  int rc = 0;
+  if (B > 0) rc = build_levels(ctx, e.depth, e.dl1, B, lvb, st, B_LEVELS);
+  if (!rc && N > 0) rc = build_levels(ctx, e.sx, e.sl1, N, lvs, st, B_LEVELS_S);
+  if (rc) return rc;
+  if (B > 0) {
+    ctx->prof_begin(S_BRACKET_MATCH, st);
+    bracket_match<BATCH><<<grid_for(B, 256), 256, 0, st>>>(e.br, B, lvb, flags, a);
+    ctx->prof_end(S_BRACKET_MATCH, st);
+    ctx->launches++;
+  }
+  if (N > 0) {
+    ctx->prof_begin(S_PARSE_PARENTS, st);
+    parse_parents<BATCH><<<grid_for(N, 256), 256, 0, st>>>(N, lvs, parent, a);
+    ctx->prof_end(S_PARSE_PARENTS, st);
+    ctx->launches++;
+  }
+  CK(hipGetLastError());
+  return 0;
+}
+
 template <typename S>
 int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, const S *ends,
                      uint64_t T, uint32_t *parent, uint32_t *node, S *span, uint64_t cap,
@@ -2257,16 +2317,9 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, co
   ctx->prof_end(S_PARSE_EXPAND, st);
   ctx->launches++;
   CK(hipGetLastError());
-  if (B > 0) {
-    Levels lv;
-    rc = build_levels(ctx, e.depth, e.dl1, B, lv, st);
-    if (rc) return rc;
-    ctx->prof_begin(S_BRACKET_MATCH, st);
-    bracket_match<false><<<grid_for(B, 256), 256, 0, st>>>(e.br, B, lv, flags, a);
-    ctx->prof_end(S_BRACKET_MATCH, st);
-    ctx->launches++;
-    CK(hipGetLastError());
-  }
+  rc = run_search<false>(ctx, a, e, B, write_nodes ? N : 0, parent, flags, st);
+  if (rc) return rc;
+  ctx->prof_end(S_PARSE, st);
   if (!write_nodes || N == 0) {
     CK(hipMemcpyAsync(ctx->h_small, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
@@ -2277,15 +2330,6 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, co
     }
     return ALPACC_ACCEPT;
   }
-  Levels lv;
-  rc = build_levels(ctx, e.sx, e.sl1, N, lv, st);
-  if (rc) return rc;
-  ctx->prof_begin(S_PARSE_PARENTS, st);
-  parse_parents<false><<<grid_for(N, 256), 256, 0, st>>>(N, lv, parent, a);
-  ctx->prof_end(S_PARSE_PARENTS, st);
-  ctx->launches++;
-  CK(hipGetLastError());
-  ctx->prof_end(S_PARSE, st);
   CK(hipMemcpyAsync(ctx->h_small, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   CK(hipStreamSynchronize(st));
   if (ctx->h_small[0]) return ALPACC_REJECT;  // brackets_matches (parser.fut:174-184)
@@ -2417,24 +2461,8 @@ int run_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offset
     ctx->prof_end(S_PARSE_EXPAND, st);
     ctx->launches++;
   }
-  if (B > 0) {
-    Levels lv;
-    rc = build_levels(ctx, e.depth, e.dl1, B, lv, st);
-    if (rc) return rc;
-    ctx->prof_begin(S_BRACKET_MATCH, st);
-    bracket_match<true><<<grid_for(B, 256), 256, 0, st>>>(e.br, B, lv, a.flags, a);
-    ctx->prof_end(S_BRACKET_MATCH, st);
-    ctx->launches++;
-  }
-  if (write_nodes && N > 0) {
-    Levels lv;
-    rc = build_levels(ctx, e.sx, e.sl1, N, lv, st);
-    if (rc) return rc;
-    ctx->prof_begin(S_PARSE_PARENTS, st);
-    parse_parents<true><<<grid_for(N, 256), 256, 0, st>>>(N, lv, parent, a);
-    ctx->prof_end(S_PARSE_PARENTS, st);
-    ctx->launches++;
-  }
+  rc = run_search<true>(ctx, a, e, B, write_nodes ? N : 0, parent, a.flags, st);
+  if (rc) return rc;
   pb_finish<<<(ndocs + 255) / 256, 256, 0, st>>>(a, d_results);
   ctx->launches++;
   CK(hipGetLastError());

@@ -60,8 +60,8 @@ class Request:
 class Server:
     """Continuous batching of parse requests on one GPU."""
 
-    def __init__(self, bundle, workers: int = 2, hbm_budget: float = 200e9,
-                 max_batch_docs: int = 4096, max_batch_bytes: int = 1 << 30, device: int = 0):
+    def __init__(self, bundle, workers: int = 4, hbm_budget: float = 200e9,
+                 max_batch_docs: int = 4096, max_batch_bytes: int = 256 << 20, device: int = 0):
         self.bundle = bundle
         self.budget = int(hbm_budget)
         self.max_docs = max_batch_docs
@@ -129,8 +129,7 @@ class Server:
                 ctx = Context(self.bundle, self.device)
             stream = lib.alpacc_stream_create()
             ctx.stream = stream
-            state = {"in": None, "pin_in": None, "out": None, "pin_out": None, "cap": 0,
-                     "ratio": 1.0}
+            state = {"in": None, "pin_in": None, "out": None, "ratio": 1.0}
         except Exception as exc:  # reported to the constructor
             self._errors.append(exc)
             self._ready.wait()
@@ -166,9 +165,8 @@ class Server:
         for k in ("in", "out"):
             for b in (state[k] or []):
                 b.free()
-        for k in ("pin_in", "pin_out"):
-            if state[k] is not None:
-                state[k].free()
+        if state["pin_in"] is not None:
+            state["pin_in"].free()
         ctx.close()
         lib.alpacc_stream_destroy(stream)
 
@@ -213,22 +211,20 @@ class Server:
             break
         if total:
             state["ratio"] = max(state["ratio"] * 0.9, nodes / total)
-        pout = self._grow_pin(state, "pin_out", 16 * max(nodes, 1), lib)
+        # the batch's trees come back into fresh arrays; every request gets views of its range
+        par = np.empty(nodes, dtype=np.uint32)
+        nd = np.empty(nodes, dtype=np.uint32)
+        sp = np.empty((nodes, 2), dtype=np.uint32)
         if nodes:
-            for k, b in enumerate(outs):
-                width = 8 if k == 2 else 4
-                off = 0 if k == 0 else (4 * nodes if k == 1 else 8 * nodes)
-                if lib.alpacc_copy_to_host(pout.ptr + off, b.ptr, width * nodes, stream) != 0:
+            for dst, b in ((par, outs[0]), (nd, outs[1]), (sp, outs[2])):
+                if lib.alpacc_copy_to_host(dst.ctypes.data, b.ptr, dst.nbytes, stream) != 0:
                     raise RuntimeError("device to host copy failed")
-        par = pout.array[:4 * nodes].view(np.uint32)
-        nd = pout.array[4 * nodes:8 * nodes].view(np.uint32)
-        sp = pout.array[8 * nodes:16 * nodes].view(np.uint32).reshape(-1, 2)
         now = time.perf_counter()
         for r, x in zip(batch, res):
             r.status = int(x["status"])
             if r.status == ACCEPT:
                 o, c = int(x["offset"]), int(x["count"])
-                r.tree = (par[o:o + c].copy(), nd[o:o + c].copy(), sp[o:o + c].copy())
+                r.tree = (par[o:o + c], nd[o:o + c], sp[o:o + c])
             r.batch = launch
             r.t_done = now
             r.done.set()

@@ -48,7 +48,7 @@ def parse_args():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=400.0, help="offered load, requests/s")
-    ap.add_argument("--serve-workers", type=int, default=3)
+    ap.add_argument("--serve-workers", type=int, default=4)
     return ap.parse_args()
 
 
@@ -286,6 +286,18 @@ def main():
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbs, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
             "traffic": None, "avg_ms": round(stage_ms[dom], 4), "algorithmic_bytes": int(alg[dom])}
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of this
+    # workload (scaled by a plain copy of known size); only for the workload they were taken on
+    tpath = os.path.join(ROOT, "profiles", "json_lex_parse_256MiB.traffic.json")
+    kernel_of = {"lex_reduce": "lex_reduce", "lex_emit": "lex_emit", "parse_expand": "parse_expand2",
+                 "bracket_match": "bracket_match", "parse_parents": "parse_parents"}
+    if (args.workload == "parse" and args.mib == 256 and args.seed == 42
+            and os.path.exists(tpath)):
+        tk = json.load(open(tpath))["kernels"].get(kernel_of.get(dom, ""))
+        if tk:
+            roof["traffic"] = tk["traffic_bytes"]
+            roof["traffic_source"] = ("profiles/json_lex_parse_256MiB.traffic.json: rocprofv3 "
+                                      "FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a 256 MiB copy")
     lex_ms = sum(stage_ms.get(k, 0.0) for k in ("lex_reduce", "lex_scan", "lex_emit"))
     dev_ms = lex_ms + stage_ms.get("parse", 0.0)
     pipe_bytes = n + 16 * count if args.workload == "parse" else n + 12 * T
