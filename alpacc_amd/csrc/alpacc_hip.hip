@@ -109,6 +109,13 @@ __device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
   return __builtin_amdgcn_readlane(wave_incl_i32(v, INT32_MAX, OpMin()), WAVE - 1);
 }
 
+// orders LDS writes of some lanes before reads by other lanes of the same wavefront
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct LexArgs {
   const uint8_t *in;
   uint64_t n;
@@ -1443,13 +1450,16 @@ __global__ void pb_docs(PArgs a) {
   a.pdocs[d] = pd;
 }
 
-constexpr int PX_STAGE = 256;  // elements staged per wavefront and round
+constexpr int PX_STAGE = 256;  // element slots per wavefront and round
+constexpr size_t PX_TABLE_LDS_MAX = 40 << 10;  // stackw + prodw, copied to LDS per block
 
 // Per tile: the positions' prefixes (block scan), then the brackets and productions of every
-// wavefront's positions. A lane's 4 positions own one contiguous run of elements; each round
-// stages the elements of a 64-aligned window of PX_STAGE in LDS and copies them out coalesced,
-// with the minimum of every 64-element chunk (chunks shared with a neighbouring wavefront go
-// through atomicMin on an array cleared to INT32_MAX).
+// wavefront's positions, written coalesced: each round covers a 64-aligned window of PX_STAGE
+// element slots; every position whose elements begin inside the window marks its first slot in
+// LDS, and a running max-scan over the marks gives every slot its position (a load-balanced
+// expansion: one lane per element, whatever the positions' element counts). Chunks of 64 elements
+// shared with a neighbouring wavefront take their minimum through atomicMin on an array cleared to
+// INT32_MAX.
 template <typename S, bool BATCH>
 __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *starts, const S *ends,
                                                             int32_t *depth, uint16_t *br,
@@ -1457,20 +1467,17 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
                                                             uint32_t *node, S *span,
                                                             int write_nodes) {
   __shared__ PTup s_w[PK_THREADS / WAVE];
-  __shared__ uint32_t s_stage[PK_THREADS / WAVE][3][PX_STAGE];
+  __shared__ uint4 s_pos[PK_THREADS / WAVE][WAVE * PK_IPT];   // per position of the wavefront
+  __shared__ __align__(16) uint16_t s_mark[PK_THREADS / WAVE][PX_STAGE];    // 1 + position starting at a slot
   extern __shared__ __align__(16) uint8_t px_smem[];  // stackw, then prodw
   uint32_t *s_stackw = reinterpret_cast<uint32_t *>(px_smem);
   uint2 *s_prodw = reinterpret_cast<uint2 *>(px_smem + 4 * ((a.nstack + 1) & ~1u));
-  for (uint32_t i = threadIdx.x; i < a.nstack; i += blockDim.x) s_stackw[i] = a.stackw[i];
-  for (uint32_t i = threadIdx.x; i < a.nprod; i += blockDim.x) s_prodw[i] = a.prodw[i];
   const int lane = threadIdx.x & (WAVE - 1);
   const int w = threadIdx.x / WAVE;
   const uint64_t tile = blockIdx.x;
   if (BATCH && tile >= *a.ntiles_dev) return;  // the grid covers the largest possible tile count
-  __syncthreads();  // tables in LDS
   const PosView pv = pos_view<BATCH>(a, tile);
   const int subst = pv.subst;
-  const PTup pre = a.pre[tile];
   const uint64_t pbase = tile * PK_TILE + (uint64_t)threadIdx.x * PK_IPT;  // slot
   const uint64_t pend = tile * PK_TILE + (pv.P > (uint64_t)pv.p0 ? pv.P - (uint64_t)pv.p0 : 0);
   uint16_t ids[PK_IPT] = {0, 0, 0, 0};
@@ -1484,6 +1491,10 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
     for (int i = 0; i < PK_IPT; ++i)
       if (pbase + i < pend) ids[i] = a.ent[pbase + i];
   }
+  // the tables' copy into LDS overlaps the loads of the entries above
+  for (uint32_t i = threadIdx.x; i < a.nstack; i += blockDim.x) s_stackw[i] = a.stackw[i];
+  for (uint32_t i = threadIdx.x; i < a.nprod; i += blockDim.x) s_prodw[i] = a.prodw[i];
+  const PTup pre = a.pre[tile];
   EntInfo inf[PK_IPT];
   PTup acc = ptup_id();
 #pragma unroll
@@ -1493,60 +1504,53 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
     acc = ptup_op(acc, ptup_of(inf[i]));
   }
   PTup total;
-  const PTup ex = ptup_block_excl(acc, s_w, total);  // also orders the table loads above
+  const PTup ex = ptup_block_excl(acc, s_w, total);  // also orders the table copies above
   // element prefixes of this lane's positions (global)
-  int32_t ob[PK_IPT], op[PK_IPT], d0[PK_IPT], s0[PK_IPT], r0[PK_IPT];
+  int32_t ob[PK_IPT], op[PK_IPT];
+  uint4 *const pos = s_pos[w];
+  uint16_t *const mark = s_mark[w];
   {
-    int32_t b = pre.nb + ex.nb, q = pre.np + ex.np, d = pre.dd + ex.dd, sv = pre.ds + ex.ds,
-            r = pre.nt + ex.nt;
+    int32_t b = pre.nb + ex.nb, q = pre.np + ex.np, d = pre.dd + ex.dd;
 #pragma unroll
     for (int i = 0; i < PK_IPT; ++i) {
       ob[i] = b;
       op[i] = q;
-      d0[i] = d;
-      s0[i] = sv;
-      r0[i] = r;
+      pos[lane * PK_IPT + i] = make_uint4((uint32_t)b, (uint32_t)d, inf[i].sst, 0u);
       b += inf[i].nb;
       q += inf[i].np;
       d += inf[i].dd;
-      sv += inf[i].ds;
-      r += inf[i].nt;
     }
   }
   const int32_t lb_hi = ob[PK_IPT - 1] + inf[PK_IPT - 1].nb;  // this lane's brackets: [ob[0], lb_hi)
   const int32_t lp_hi = op[PK_IPT - 1] + inf[PK_IPT - 1].np;
-  uint32_t *st0 = s_stage[w][0];
-  uint32_t *st1 = s_stage[w][1];
-  uint32_t *st2 = s_stage[w][2];
 
   // brackets: depth value (depths, parser.fut:153-166) and symbol
   {
     const int32_t wlo = __shfl(ob[0], 0, WAVE), whi = __shfl(lb_hi, WAVE - 1, WAVE);
+    int32_t carry = 0;
     for (int32_t r0w = wlo & ~(WAVE - 1); r0w < whi; r0w += PX_STAGE) {
-      const int32_t glo = max(ob[0], r0w), ghi = min(lb_hi, r0w + PX_STAGE);
-      for (int32_t g = glo; g < ghi; ++g) {
-        const int i = g >= ob[3] ? 3 : g >= ob[2] ? 2 : g >= ob[1] ? 1 : 0;
-        const int32_t obi = i == 3 ? ob[3] : i == 2 ? ob[2] : i == 1 ? ob[1] : ob[0];
-        const int32_t di = i == 3 ? d0[3] : i == 2 ? d0[2] : i == 1 ? d0[1] : d0[0];
-        const uint32_t ssti = i == 3 ? inf[3].sst : i == 2 ? inf[2].sst : i == 1 ? inf[1].sst : inf[0].sst;
-        const uint32_t sw = s_stackw[ssti + (uint32_t)(g - obi)];
-        st0[g - r0w] = (uint32_t)(di + (int32_t)(int16_t)(sw >> 16));
-        st1[g - r0w] = sw & 0xFFFFu;
-      }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      reinterpret_cast<uint2 *>(mark)[lane] = make_uint2(0u, 0u);
+      wave_lds_sync();
+#pragma unroll
+      for (int i = 0; i < PK_IPT; ++i)
+        if (inf[i].nb && ob[i] >= r0w && ob[i] < r0w + PX_STAGE)
+          mark[ob[i] - r0w] = (uint16_t)(lane * PK_IPT + i + 1);
+      wave_lds_sync();
 #pragma unroll
       for (int j = 0; j < PX_STAGE / WAVE; ++j) {
         const int32_t cb = r0w + j * WAVE;
         if (cb >= whi) break;
         const int32_t g = cb + lane;
+        const int32_t o = max(wave_incl_i32((int32_t)mark[j * WAVE + lane], 0, OpMax()), carry);
+        carry = __builtin_amdgcn_readlane(o, WAVE - 1);
         const bool act = g >= wlo && g < whi;
         int32_t dv = INT32_MAX;
         if (act) {
-          dv = (int32_t)st0[j * WAVE + lane];
+          const uint4 p = pos[o - 1];
+          const uint32_t sw = s_stackw[p.z + (uint32_t)(g - (int32_t)p.x)];
+          dv = (int32_t)p.y + (int32_t)(int16_t)(sw >> 16);
           depth[g] = dv;
-          br[g] = (uint16_t)st1[j * WAVE + lane];
+          br[g] = (uint16_t)sw;
         }
         const int32_t m = wave_min_i32(dv);
         if (lane == 0) {
@@ -1554,68 +1558,85 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
           else atomicMin(dl1 + cb / WAVE, m);
         }
       }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      wave_lds_sync();
+    }
+  }
+  {
+    int32_t q = pre.np + ex.np, sv = pre.ds + ex.ds, r = pre.nt + ex.nt;
+#pragma unroll
+    for (int i = 0; i < PK_IPT; ++i) {
+      pos[lane * PK_IPT + i] = make_uint4((uint32_t)q, (uint32_t)sv, (uint32_t)r, inf[i].pst);
+      q += inf[i].np;
+      sv += inf[i].ds;
+      r += inf[i].nt;
     }
   }
   // productions: s (parents, parser.fut:286-296), node and span (terminal_offsets, :305-332)
   {
     const int32_t wlo = __shfl(op[0], 0, WAVE), whi = __shfl(lp_hi, WAVE - 1, WAVE);
+    int32_t carry = 0;
     for (int32_t r0w = wlo & ~(WAVE - 1); r0w < whi; r0w += PX_STAGE) {
-      const int32_t glo = max(op[0], r0w), ghi = min(lp_hi, r0w + PX_STAGE);
-      for (int32_t g = glo; g < ghi; ++g) {
-        const int i = g >= op[3] ? 3 : g >= op[2] ? 2 : g >= op[1] ? 1 : 0;
-        const int32_t opi = i == 3 ? op[3] : i == 2 ? op[2] : i == 1 ? op[1] : op[0];
-        const int32_t si = i == 3 ? s0[3] : i == 2 ? s0[2] : i == 1 ? s0[1] : s0[0];
-        const int32_t ri = i == 3 ? r0[3] : i == 2 ? r0[2] : i == 1 ? r0[1] : r0[0];
-        const uint32_t psti = i == 3 ? inf[3].pst : i == 2 ? inf[2].pst : i == 1 ? inf[1].pst : inf[0].pst;
-        const uint2 pw = s_prodw[psti + (uint32_t)(g - opi)];
-        const int32_t t = subst ? (int32_t)(pw.x >> 16) - 1 : -1;
-        st0[g - r0w] = (uint32_t)(si + (int32_t)(int16_t)(pw.y & 0xFFFFu));
-        st1[g - r0w] = t >= 0 ? (ALPACC_NODE_TERMINAL | (uint32_t)t) : (pw.x & 0xFFFFu);
-        st2[g - r0w] = t >= 0 ? (uint32_t)pv.tok_base + (uint32_t)(ri + (int32_t)(pw.y >> 16))
-                              : 0xFFFFFFFFu;
-      }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      reinterpret_cast<uint2 *>(mark)[lane] = make_uint2(0u, 0u);
+      wave_lds_sync();
 #pragma unroll
-      for (int j = 0; j < PX_STAGE / WAVE; ++j) {
+      for (int i = 0; i < PK_IPT; ++i)
+        if (inf[i].np && op[i] >= r0w && op[i] < r0w + PX_STAGE)
+          mark[op[i] - r0w] = (uint16_t)(lane * PK_IPT + i + 1);
+      wave_lds_sync();
+      // every chunk's values and span loads first, then the stores: the loads of later chunks
+      // do not wait behind the stores of earlier ones
+      constexpr int NJ = PX_STAGE / WAVE;
+      int32_t svs[NJ];
+      uint32_t nds[NJ];
+      S v0[NJ], v1[NJ];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int32_t cb = r0w + j * WAVE;
+        const int32_t g = cb + lane;
+        const int32_t o = max(wave_incl_i32((int32_t)mark[j * WAVE + lane], 0, OpMax()), carry);
+        carry = __builtin_amdgcn_readlane(o, WAVE - 1);
+        svs[j] = INT32_MAX;
+        nds[j] = 0;
+        v0[j] = 0;
+        v1[j] = 0;
+        if (g >= wlo && g < whi) {
+          const uint4 p = pos[o - 1];
+          const uint2 pw = s_prodw[p.w + (uint32_t)(g - (int32_t)p.x)];
+          const int32_t t = subst ? (int32_t)(pw.x >> 16) - 1 : -1;
+          svs[j] = (int32_t)p.y + (int32_t)(int16_t)(pw.y & 0xFFFFu);
+          nds[j] = t >= 0 ? (ALPACC_NODE_TERMINAL | (uint32_t)t) : (pw.x & 0xFFFFu);
+          if (write_nodes && t >= 0) {
+            const uint64_t rk = pv.tok_base + (uint32_t)((int32_t)p.z + (int32_t)(pw.y >> 16));
+            v0[j] = starts[rk];
+            v1[j] = ends[rk];
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
         const int32_t cb = r0w + j * WAVE;
         if (cb >= whi) break;
         const int32_t g = cb + lane;
-        const bool act = g >= wlo && g < whi;
-        int32_t sv = INT32_MAX;
-        if (act) {
-          sv = (int32_t)st0[j * WAVE + lane];
-          sx[g] = sv;
+        if (g >= wlo && g < whi) {
+          sx[g] = svs[j];
           if (write_nodes) {
-            const uint32_t r = st2[j * WAVE + lane];
-            S v0 = 0, v1 = 0;
-            if (r != 0xFFFFFFFFu) {
-              v0 = starts[r];
-              v1 = ends[r];
-            }
-            node[g] = st1[j * WAVE + lane];
+            node[g] = nds[j];
             if (sizeof(S) == 4) {
-              reinterpret_cast<uint2 *>(span)[g] = make_uint2((uint32_t)v0, (uint32_t)v1);
+              reinterpret_cast<uint2 *>(span)[g] = make_uint2((uint32_t)v0[j], (uint32_t)v1[j]);
             } else {
-              span[2 * (uint64_t)g] = v0;
-              span[2 * (uint64_t)g + 1] = v1;
+              span[2 * (uint64_t)g] = v0[j];
+              span[2 * (uint64_t)g + 1] = v1[j];
             }
           }
         }
-        const int32_t m = wave_min_i32(sv);
+        const int32_t m = wave_min_i32(svs[j]);
         if (lane == 0) {
           if (cb >= wlo && cb + WAVE <= whi) sl1[cb / WAVE] = m;
           else atomicMin(sl1 + cb / WAVE, m);
           if (BATCH && cb >= wlo) a.chunk_doc[cb / WAVE] = pv.doc;
         }
       }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      wave_lds_sync();
     }
   }
 }
@@ -2551,6 +2572,10 @@ bool build_parser_tables(alpacc_ctx *ctx, const alpacc_tables *t, std::string &e
   }
   if (stackw.empty()) stackw.push_back(0);
   if (prodw.empty()) prodw.push_back(make_uint2(0, 0));
+  if (4 * ((stackw.size() + 1) & ~(size_t)1) + 8 * prodw.size() > PX_TABLE_LDS_MAX) {
+    err = "bracket and production tables too large for LDS";
+    return false;
+  }
   ctx->index_len = t->pair_index_len;
   ctx->d_pair_index = upload(t->pair_index, t->pair_index_len, err);
   if (E < 255 && t->pair_index_len <= (uint64_t)PK_IDX8_MAX) {
