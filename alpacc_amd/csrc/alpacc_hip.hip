@@ -919,21 +919,41 @@ __device__ __forceinline__ int lift_left(int32_t x, int pos, int32_t v) {
   return pos;
 }
 
-// Minimum of the 64 chunks before `chunk` (lane l: chunk - 64 + l), the window the far search uses.
-__device__ __forceinline__ int32_t window_min(const Levels &lv, uint64_t chunk, int lane) {
-  if (lv.nlev < 2) return INT32_MAX;
-  const int64_t wc = (int64_t)chunk - WAVE + lane;
-  return wc >= 0 ? lv.L[1][wc] : INT32_MAX;
+// Last j with y_j <= v among 64 values held in reverse (lane l holds y_{63-l}; INT32_MAX for
+// missing values), or -1: the prefix minima of the reversed values fall monotonically with the
+// lane, so the answer is 63 minus the number of prefix minima above v (a 6-step binary search).
+// All 64 lanes must call this together; a lane that asks nothing passes v = INT32_MIN.
+__device__ __forceinline__ int last_le_rev(int32_t yrev, int32_t v) {
+  const int32_t pm = wave_incl_i32(yrev, INT32_MAX, OpMin());
+  const int32_t lowest = __builtin_amdgcn_readlane(pm, WAVE - 1);
+  int lo = 0;
+#pragma unroll
+  for (int step = WAVE / 2; step >= 1; step >>= 1)
+    if (__shfl(pm, lo + step - 1, WAVE) > v) lo += step;
+  return lowest <= v ? WAVE - 1 - lo : -1;
+}
+
+// The reversed inputs of the search for chunk `chunk`: the previous chunk's elements (lane l:
+// element 64*chunk - 1 - l) and the minima of the 64 chunks before that (lane l: chunk - 2 - l).
+struct SearchRev {
+  int32_t prev, win;
+};
+
+__device__ __forceinline__ SearchRev search_rev(const Levels &lv, uint64_t chunk, int lane) {
+  SearchRev r{INT32_MAX, INT32_MAX};
+  if (chunk >= 1) r.prev = lv.L[0][chunk * WAVE - 1 - lane];
+  if (lv.nlev >= 2 && (int64_t)chunk - 2 - lane >= 0) r.win = lv.L[1][chunk - 2 - lane];
+  return r;
 }
 
 // Wave-cooperative previous smaller-or-equal for the 64 consecutive elements of chunk `chunk`
-// (lane l holds element chunk*64+l with value v; lanes past the end hold INT32_MAX; wv is
-// window_min of the chunk): binary lifting inside the chunk; then binary lifting over the window
-// picks the chunk holding the answer, and each distinct such chunk is loaded once (coalesced)
-// and searched by lifting again; a per-lane climb of the min hierarchy handles the rest.
-// All 64 lanes must call this together.
-__device__ int64_t prev_le_wave(const Levels &lv, uint64_t chunk, int lane, int32_t v, int32_t wv,
-                                bool need) {
+// (lane l holds element chunk*64+l with value v; lanes past the end hold INT32_MAX): binary
+// lifting inside the chunk; then the previous chunk (every chunk before the last one is full);
+// then a search over the minima of the 64 chunks before that picks the chunk holding the answer,
+// and each distinct such chunk is loaded once (coalesced) and searched; a per-lane climb of the
+// min hierarchy handles the rest. All 64 lanes must call this together.
+__device__ int64_t prev_le_wave(const Levels &lv, uint64_t chunk, int lane, int32_t v,
+                                const SearchRev &rv, bool need) {
   const int64_t cbase = (int64_t)(chunk * WAVE);
   const int p0 = lift_left(v, lane, v);
   int64_t res = -1;
@@ -942,28 +962,33 @@ __device__ int64_t prev_le_wave(const Levels &lv, uint64_t chunk, int lane, int3
     res = cbase + p0 - 1;
     done = true;
   }
-  if (__ballot(!done) == 0ull) return res;
-  if (lv.nlev >= 2) {
-    const int p1 = lift_left(wv, WAVE, v);
-    // target chunk of every unresolved lane (-1: not in the window)
-    int64_t cb = (!done && p1 > 0) ? (int64_t)chunk - WAVE + p1 - 1 : -1;
-    uint64_t pend = __ballot(cb >= 0);
-    while (pend) {
-      const int t = __ffsll((long long)pend) - 1;
-      const int64_t c = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cb & 0xFFFFFFFF), t) |
-                        ((int64_t)__builtin_amdgcn_readlane((int)(cb >> 32), t) << 32);
-      const int32_t x = lv.L[0][c * WAVE + lane];
-      const bool mine = cb == c;
-      const int p2 = lift_left(x, WAVE, mine ? v : INT32_MIN);
-      if (mine) {
-        res = c * WAVE + p2 - 1;
-        done = true;
-        cb = -1;
-      }
-      pend = __ballot(cb >= 0);
-    }
+  if (__ballot(!done) == 0ull || chunk == 0) return res;
+  const int j1 = last_le_rev(rv.prev, done ? INT32_MIN : v);
+  if (!done && j1 >= 0) {
+    res = cbase - WAVE + j1;
+    done = true;
   }
-  if (!done) res = prev_le_far(lv, chunk > (uint64_t)WAVE ? chunk - WAVE : 0, v);
+  if (__ballot(!done) == 0ull || chunk == 1) return res;
+  // window: chunks chunk-65 .. chunk-2 (window index j is chunk chunk-65+j)
+  const int jw = last_le_rev(rv.win, done ? INT32_MIN : v);
+  int64_t cb = (!done && jw >= 0) ? (int64_t)chunk - (WAVE + 1) + jw : -1;
+  uint64_t pend = __ballot(cb >= 0);
+  while (pend) {
+    const int t = __ffsll((long long)pend) - 1;
+    const int64_t c = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cb & 0xFFFFFFFF), t) |
+                      ((int64_t)__builtin_amdgcn_readlane((int)(cb >> 32), t) << 32);
+    const int32_t yrev = lv.L[0][c * WAVE + (WAVE - 1) - lane];
+    const bool mine = cb == c;
+    const int j2 = last_le_rev(yrev, mine ? v : INT32_MIN);
+    if (mine) {
+      res = c * WAVE + j2;
+      done = true;
+      cb = -1;
+    }
+    pend = __ballot(cb >= 0);
+  }
+  if (!done)
+    res = prev_le_far(lv, chunk > (uint64_t)(WAVE + 1) ? chunk - (WAVE + 1) : 0, v);
   return res;
 }
 
@@ -1662,9 +1687,9 @@ __device__ void bracket_match_run(const uint16_t *br, uint64_t B, const Levels &
     const bool valid = i < B;
     const uint16_t b = valid ? br[i] : (uint16_t)0x8000u;
     const int32_t v = valid ? lv.L[0][i] : INT32_MAX;
-    const int32_t wv = window_min(lv, c, lane);
+    const SearchRev rv = search_rev(lv, c, lane);
     const bool right = valid && !(b & 0x8000u);
-    const int64_t j = prev_le_wave(lv, c, lane, v, wv, right);
+    const int64_t j = prev_le_wave(lv, c, lane, v, rv, right);
     const int64_t jl = j - (int64_t)(c * WAVE);
     const uint16_t bin = (uint16_t)__shfl((int)b, (int)(jl & (WAVE - 1)), WAVE);
     bool mism = false;
@@ -1711,8 +1736,8 @@ __device__ void parents_run(uint64_t N, const Levels &lv, uint32_t *parent, cons
       base = a.pdocs[d].node_off;
     }
     const int32_t v = valid ? lv.L[0][i] : INT32_MAX;
-    const int32_t wv = window_min(lv, c, lane);
-    const int64_t j = prev_le_wave(lv, c, lane, v, wv, valid && i > base);
+    const SearchRev rv = search_rev(lv, c, lane);
+    const int64_t j = prev_le_wave(lv, c, lane, v, rv, valid && i > base);
     if (valid) parent[i] = i == base ? 0u : (uint32_t)((uint64_t)j - base);
   }
 }
