@@ -896,23 +896,29 @@ __device__ __forceinline__ int hibit64(uint64_t m) { return 63 - __clzll((long l
 // Binary lifting over a wave-level sparse table of minima: lane l holds x_l; returns for every lane
 // the smallest p in [0, pos] such that x_p .. x_{pos-1} are all > v (p = pos when x_{pos-1} <= v).
 // All 64 lanes must call this together.
+//
+// ds_bpermute selects the source lane by address bits 7:2 and reads no memory, so lane indexes
+// are kept as byte addresses and a negative one merely reads some lane whose value goes unused.
 __device__ __forceinline__ int lift_left(int32_t x, int pos, int32_t v) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  // M[k] at lane l: min of x_l .. x_{l+2^k-1}. Windows running past lane 63 wrap around and hold
+  // wrong minima, but no query below reads a window that ends past x_63.
   int32_t M[6];
   M[0] = x;
 #pragma unroll
   for (int k = 1; k < 6; ++k) {
-    const int32_t o = __shfl_down(M[k - 1], 1 << (k - 1), WAVE);
-    const int lane = threadIdx.x & (WAVE - 1);
-    M[k] = (lane + (1 << (k - 1)) < WAVE) ? min(M[k - 1], o) : M[k - 1];
+    const int s = 1 << (k - 1);
+    M[k] = min(M[k - 1], __builtin_amdgcn_ds_bpermute(((lane + s) & (WAVE - 1)) << 2, M[k - 1]));
   }
+  int p4 = pos << 2;
 #pragma unroll
   for (int k = 5; k >= 0; --k) {
-    const int step = 1 << k;
-    const int src = pos - step;
+    const int src4 = p4 - (4 << k);
     // M[k] at lane src covers x_src .. x_{src+step-1} = x_{pos-step} .. x_{pos-1}
-    const int32_t m = __shfl(M[k], src < 0 ? 0 : src, WAVE);
-    if (src >= 0 && m > v) pos = src;
+    const int32_t m = __builtin_amdgcn_ds_bpermute(src4, M[k]);
+    if (src4 >= 0 && m > v) p4 = src4;
   }
+  pos = p4 >> 2;
   // the steps add up to 63: from pos = 64 the loop stops at 1 without having looked at x_0
   const int32_t x0 = __builtin_amdgcn_readlane(x, 0);
   if (pos == 1 && x0 > v) pos = 0;
@@ -926,11 +932,11 @@ __device__ __forceinline__ int lift_left(int32_t x, int pos, int32_t v) {
 __device__ __forceinline__ int last_le_rev(int32_t yrev, int32_t v) {
   const int32_t pm = wave_incl_i32(yrev, INT32_MAX, OpMin());
   const int32_t lowest = __builtin_amdgcn_readlane(pm, WAVE - 1);
-  int lo = 0;
+  int lo4 = 0;  // 4 * the number of prefix minima known to lie above v
 #pragma unroll
   for (int step = WAVE / 2; step >= 1; step >>= 1)
-    if (__shfl(pm, lo + step - 1, WAVE) > v) lo += step;
-  return lowest <= v ? WAVE - 1 - lo : -1;
+    if (__builtin_amdgcn_ds_bpermute(lo4 + 4 * (step - 1), pm) > v) lo4 += 4 * step;
+  return lowest <= v ? WAVE - 1 - (lo4 >> 2) : -1;
 }
 
 // The reversed inputs of the search for chunk `chunk`: the previous chunk's elements (lane l:
