@@ -1482,6 +1482,7 @@ __global__ void pb_docs(PArgs a) {
 }
 
 constexpr int PX_STAGE = 256;  // element slots per wavefront and round
+constexpr int PX_TOK = 320;    // token spans per wavefront staged in LDS (32-bit spans)
 constexpr size_t PX_TABLE_LDS_MAX = 40 << 10;  // stackw + prodw, copied to LDS per block
 
 // Per tile: the positions' prefixes (block scan), then the brackets and productions of every
@@ -1500,6 +1501,9 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
   __shared__ PTup s_w[PK_THREADS / WAVE];
   __shared__ uint4 s_pos[PK_THREADS / WAVE][WAVE * PK_IPT];   // per position of the wavefront
   __shared__ __align__(16) uint16_t s_mark[PK_THREADS / WAVE][PX_STAGE];    // 1 + position starting at a slot
+  // the spans of the tokens the wavefront's terminal productions take (a contiguous rank range)
+  constexpr int NTOK = sizeof(S) == 4 ? PX_TOK : 1;
+  __shared__ uint2 s_tok[PK_THREADS / WAVE][NTOK];
   extern __shared__ __align__(16) uint8_t px_smem[];  // stackw, then prodw
   uint32_t *s_stackw = reinterpret_cast<uint32_t *>(px_smem);
   uint2 *s_prodw = reinterpret_cast<uint2 *>(px_smem + 4 * ((a.nstack + 1) & ~1u));
@@ -1554,6 +1558,33 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
   }
   const int32_t lb_hi = ob[PK_IPT - 1] + inf[PK_IPT - 1].nb;  // this lane's brackets: [ob[0], lb_hi)
   const int32_t lp_hi = op[PK_IPT - 1] + inf[PK_IPT - 1].np;
+  // ranks of the wavefront's terminal productions: [rw_lo, rw_hi); their spans go to LDS now, so
+  // that the production rounds below read no global memory
+  int32_t rw_lo = 0;
+  bool tok_lds = false;
+  if (sizeof(S) == 4 && write_nodes && subst) {
+    int32_t rl = pre.nt + ex.nt, rh = rl;
+#pragma unroll
+    for (int i = 0; i < PK_IPT; ++i) rh += inf[i].nt;
+    rw_lo = __shfl(rl, 0, WAVE);
+    const int32_t cnt = __shfl(rh, WAVE - 1, WAVE) - rw_lo;
+    tok_lds = cnt <= PX_TOK;
+    if (tok_lds) {
+      uint32_t ts[PX_TOK / WAVE], te[PX_TOK / WAVE];
+#pragma unroll
+      for (int u = 0; u < PX_TOK / WAVE; ++u) {
+        const int32_t t = u * WAVE + lane;
+        ts[u] = te[u] = 0;
+        if (t < cnt) {
+          const uint64_t gk = pv.tok_base + (uint32_t)(rw_lo + t);
+          ts[u] = (uint32_t)starts[gk];
+          te[u] = (uint32_t)ends[gk];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < PX_TOK / WAVE; ++u) s_tok[w][u * WAVE + lane] = make_uint2(ts[u], te[u]);
+    }
+  }
 
   // brackets: depth value (depths, parser.fut:153-166) and symbol
   {
@@ -1637,9 +1668,16 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
           svs[j] = (int32_t)p.y + (int32_t)(int16_t)(pw.y & 0xFFFFu);
           nds[j] = t >= 0 ? (ALPACC_NODE_TERMINAL | (uint32_t)t) : (pw.x & 0xFFFFu);
           if (write_nodes && t >= 0) {
-            const uint64_t rk = pv.tok_base + (uint32_t)((int32_t)p.z + (int32_t)(pw.y >> 16));
-            v0[j] = starts[rk];
-            v1[j] = ends[rk];
+            const int32_t rk = (int32_t)p.z + (int32_t)(pw.y >> 16);
+            if (tok_lds) {
+              const uint2 sp = s_tok[w][rk - rw_lo];
+              v0[j] = (S)sp.x;
+              v1[j] = (S)sp.y;
+            } else {
+              const uint64_t gk = pv.tok_base + (uint32_t)rk;
+              v0[j] = starts[gk];
+              v1[j] = ends[gk];
+            }
           }
         }
       }
