@@ -216,7 +216,22 @@ def main():
         for _ in range(4):
             lib.alpacc_copy_device(dst.ptr, src.ptr, nb, None)
         lib.alpacc_stream_sync(None)
-        print(json.dumps({"copy_probe_bytes": nb, "copies": 4}), flush=True)
+        out = {"copy_probe_bytes": nb, "copies": 4}
+        # practical rates for scale: device copy (read + write) and a fill (write), 20 calls each
+        # between two host synchronisations
+        import ctypes
+        for name, fn, moved in (
+                ("copy_TBps", lambda: lib.alpacc_copy_device(dst.ptr, src.ptr, nb, None), 2 * nb),
+                ("fill_TBps", lambda: lib.alpacc_device_memset(ctypes.c_void_p(dst.ptr), 7, nb,
+                                                                None), nb)):
+            fn()
+            lib.alpacc_stream_sync(None)
+            t0 = time.perf_counter()
+            for _ in range(20):
+                fn()
+            lib.alpacc_stream_sync(None)
+            out[name] = round(moved * 20 / (time.perf_counter() - t0) / 1e12, 2)
+        print(json.dumps(out), flush=True)
         return
 
     mode = "both" if args.workload == "parse" else "lexer"
