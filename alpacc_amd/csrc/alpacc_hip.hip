@@ -509,6 +509,8 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   __shared__ int32_t s_wlast[LEX_CT_WAVES];
   __shared__ uint32_t s_wreset[LEX_CT_WAVES];
   __shared__ uint32_t s_ctile;
+  __shared__ uint32_t s_cpre;  // the count tile's look-back prefix (count, last start)
+  __shared__ int32_t s_lpre;
   __shared__ uint32_t s_stage[LEX_CT_WAVES][3][LEX_STAGE];
   const uint32_t C = p.C;
   for (uint32_t i = threadIdx.x; i < p.m1 * C; i += blockDim.x) s_dfa[i] = p.dfa_step[i];
@@ -601,6 +603,9 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
     s_wreset[w] = (tile < p.num_tiles && tv.ltile == 0) ? 1u : 0u;
   }
   __syncthreads();
+  uint32_t ct_tot = 0;  // wave 0: the count tile's own total, last start and reset flag
+  int32_t ct_last = -1;
+  bool ct_reset = false, need_lb = false;
   if (w == 0) {
     // segmented scan of the 16 wave totals (a reset keeps only what follows it)
     const uint32_t wc = lane < LEX_CT_WAVES ? s_wcnt[lane] : 0u;
@@ -629,87 +634,117 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
     const bool ftot = __builtin_amdgcn_readlane(wfi, LEX_CT_WAVES - 1) != 0;
     // a count tile holding a document start already has its (post-reset) inclusive value
     if (lane == 0) st_relaxed64(p.st_count + ct, pack_count(ctot, ltot, ftot ? CT_INC : CT_AGG));
-    uint32_t cpre = 0;
-    int32_t lpre = -1;
-    const bool need_lb = ct > 0 && __builtin_amdgcn_readlane((int)wr, 0) == 0;
-    if (need_lb) {
-      lookback_count(p, ct, lane, cpre, lpre);
-      if (lane == 0 && !ftot)
-        st_relaxed64(p.st_count + ct, pack_count(cpre + ctot, max(lpre, ltot), CT_INC));
-    }
-    // exclusive value of every wave: the inclusive value of the wave before (or nothing after a
-    // reset), plus the look-back prefix when no reset precedes it in this count tile
+    ct_tot = ctot;
+    ct_last = ltot;
+    ct_reset = ftot;
+    need_lb = ct > 0 && __builtin_amdgcn_readlane((int)wr, 0) == 0;
+    // exclusive value of every wave inside the count tile; a wave with no reset before it (its
+    // own included) adds the count tile's look-back prefix, which arrives later
     uint32_t xc = (uint32_t)wave_prev_i32(wci, 0);
     int32_t xl = wave_prev_i32(wli, -1);
     const bool xf = wave_prev_i32(wfi, 0) != 0;
     if (wr) {
       xc = 0;
       xl = -1;
-    } else if (!xf) {
-      xc += cpre;
-      xl = max(xl, lpre);
     }
     if (lane < LEX_CT_WAVES) {
       s_wcnt[lane] = xc;
       s_wlast[lane] = xl;
+      s_wreset[lane] = (wr || xf) ? 0u : 1u;
     }
   }
   __syncthreads();
 
-  // ---- write tokens: stage in LDS by wave-local index, copy out coalesced, in rounds
-  const uint32_t wout = (uint32_t)tv.tok_base + s_wcnt[w];  // slot of the wave's first token
+  // ---- write tokens: stage in LDS by wave-local index, copy out coalesced, in rounds. The first
+  // round is staged before the look-back, so that its wait overlaps the staging; a lane whose
+  // first token began before the count tile stages START_PENDING and the copy takes the
+  // look-back's last start instead.
+  constexpr uint32_t START_PENDING = 0xFFFFFFFFu;  // positions stay below 2^31
+  const bool takes_prefix = s_wreset[w] != 0;
   const uint32_t wtotal = (uint32_t)__builtin_amdgcn_readlane((int32_t)cincl, WAVE - 1);
   const uint32_t lofs = cincl - cnt;                // wave-local index of this lane's first token
   const int32_t lexcl = wave_prev_i32(lincl, -1);
-  const int32_t start_in = max(max(s_wlast[w], lexcl), 0);
+  const int32_t last_in_tile = max(s_wlast[w], lexcl);
+  uint32_t start_in = last_in_tile >= 0 ? (uint32_t)last_in_tile
+                                        : (takes_prefix ? START_PENDING : 0u);
+  uint32_t pending_start = 0;
+  uint32_t wout = 0;  // slot of the wave's first token
   uint32_t *stk = s_stage[w][0];
   uint32_t *sts = s_stage[w][1];
   uint32_t *ste = s_stage[w][2];
-  for (uint32_t r0 = 0; r0 < wtotal; r0 += LEX_STAGE) {
-    // this lane's tokens with wave-local index in [r0, r0 + LEX_STAGE)
-    uint32_t em = endm;
-    uint32_t k = lofs;
-    if (k < r0) {  // skip tokens staged in earlier rounds
-      const uint32_t skip = min(r0 - k, (uint32_t)__popc(em));
-      for (uint32_t q = 0; q < skip; ++q) em &= em - 1;
-      k += skip;
-    }
-    while (em && k < r0 + LEX_STAGE) {
-      const int j = __ffs(em) - 1;
-      em &= em - 1;
-      const uint32_t sm = startm & (j == 31 ? 0xFFFFFFFFu : ((2u << j) - 1u));
-      const int32_t cur = sm ? (int32_t)(base + 31 - __clz(sm)) : start_in;
-      uint32_t lw;
-      switch (j >> 2) {
-        case 0: lw = labs[0]; break;
-        case 1: lw = labs[1]; break;
-        case 2: lw = labs[2]; break;
-        case 3: lw = labs[3]; break;
-        case 4: lw = labs[4]; break;
-        case 5: lw = labs[5]; break;
-        case 6: lw = labs[6]; break;
-        default: lw = labs[7]; break;
+  for (uint32_t r0 = 0;; r0 += LEX_STAGE) {
+    const bool any = r0 < wtotal;  // uniform in the wave
+    if (any) {
+      // this lane's tokens with wave-local index in [r0, r0 + LEX_STAGE)
+      uint32_t em = endm;
+      uint32_t k = lofs;
+      if (k < r0) {  // skip tokens staged in earlier rounds
+        const uint32_t skip = min(r0 - k, (uint32_t)__popc(em));
+        for (uint32_t q = 0; q < skip; ++q) em &= em - 1;
+        k += skip;
       }
-      const uint32_t o = k - r0;
-      stk[o] = (lw >> (8 * (j & 3))) & 0xFFu;
-      sts[o] = (uint32_t)cur;
-      ste[o] = (uint32_t)(base + j + (CHUNKED ? 0 : 1));
-      ++k;
+      while (em && k < r0 + LEX_STAGE) {
+        const int j = __ffs(em) - 1;
+        em &= em - 1;
+        const uint32_t sm = startm & (j == 31 ? 0xFFFFFFFFu : ((2u << j) - 1u));
+        const uint32_t cur = sm ? (uint32_t)(base + 31 - __clz(sm)) : start_in;
+        uint32_t lw;
+        switch (j >> 2) {
+          case 0: lw = labs[0]; break;
+          case 1: lw = labs[1]; break;
+          case 2: lw = labs[2]; break;
+          case 3: lw = labs[3]; break;
+          case 4: lw = labs[4]; break;
+          case 5: lw = labs[5]; break;
+          case 6: lw = labs[6]; break;
+          default: lw = labs[7]; break;
+        }
+        const uint32_t o = k - r0;
+        stk[o] = (lw >> (8 * (j & 3))) & 0xFFu;
+        sts[o] = cur;
+        ste[o] = (uint32_t)(base + j + (CHUNKED ? 0 : 1));
+        ++k;
+      }
     }
+    if (r0 == 0) {
+      // the count tile's prefix: decoupled look-back by wave 0 while the staging above runs
+      if (w == 0) {
+        uint32_t cpre = 0;
+        int32_t lpre = -1;
+        if (need_lb) {
+          lookback_count(p, ct, lane, cpre, lpre);
+          if (lane == 0 && !ct_reset)
+            st_relaxed64(p.st_count + ct, pack_count(cpre + ct_tot, max(lpre, ct_last), CT_INC));
+        }
+        if (lane == 0) {
+          s_cpre = cpre;
+          s_lpre = lpre;
+        }
+      }
+      __syncthreads();
+      const uint32_t cpre = takes_prefix ? s_cpre : 0u;
+      const int32_t lpre = takes_prefix ? s_lpre : -1;
+      wout = (uint32_t)tv.tok_base + s_wcnt[w] + cpre;
+      pending_start = (uint32_t)max(lpre, 0);
+      if (start_in == START_PENDING) start_in = pending_start;
+    }
+    if (!any) break;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const uint32_t m = min((uint32_t)LEX_STAGE, wtotal - r0);
     for (uint32_t i = lane; i < m; i += WAVE) {
       const uint64_t g = (uint64_t)wout + r0 + i;
+      uint32_t sv = sts[i];
+      if (sv == START_PENDING) sv = pending_start;
       p.kinds[g] = stk[i];
       if (CHUNKED) {
         uint64_t cs, ce;
-        chunk_span(p, sts[i], ste[i], cs, ce);
+        chunk_span(p, sv, ste[i], cs, ce);
         p.starts64[g] = cs;
         p.ends64[g] = ce;
       } else {
-        p.starts[g] = sts[i];
+        p.starts[g] = sv;
         p.ends[g] = ste[i];
       }
     }
