@@ -40,7 +40,7 @@ constexpr uint32_t SPIN_LIMIT = 1u << 22;        // bounded waits: a stuck look-
 constexpr int LEX_CT_WAVES = LEX_BLOCK / WAVE;   // wave tiles per count tile (one block)
 constexpr int LEX_SCAN_BLOCK = 1024;             // wave tiles per K2a scan block
 constexpr int LEX_MAX_DFA = 4096;                // max (m+1)*C entries of the DFA table in LDS
-constexpr int LEX_STAGE = 256;                   // tokens staged per wavefront and round
+constexpr int LEX_STAGE = 384;                   // tokens staged per wavefront and round
 
 // ----------------------------------------------------------------------------------------------
 // small device helpers
@@ -511,7 +511,8 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   __shared__ uint32_t s_ctile;
   __shared__ uint32_t s_cpre;  // the count tile's look-back prefix (count, last start)
   __shared__ int32_t s_lpre;
-  __shared__ uint32_t s_stage[LEX_CT_WAVES][3][LEX_STAGE];
+  __shared__ uint32_t s_stage[LEX_CT_WAVES][2][LEX_STAGE];  // starts, ends
+  __shared__ uint8_t s_kind[LEX_CT_WAVES][LEX_STAGE];
   const uint32_t C = p.C;
   for (uint32_t i = threadIdx.x; i < p.m1 * C; i += blockDim.x) s_dfa[i] = p.dfa_step[i];
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_cls[i] = p.byte_class[i];
@@ -669,9 +670,9 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
                                         : (takes_prefix ? START_PENDING : 0u);
   uint32_t pending_start = 0;
   uint32_t wout = 0;  // slot of the wave's first token
-  uint32_t *stk = s_stage[w][0];
-  uint32_t *sts = s_stage[w][1];
-  uint32_t *ste = s_stage[w][2];
+  uint8_t *stk = s_kind[w];
+  uint32_t *sts = s_stage[w][0];
+  uint32_t *ste = s_stage[w][1];
   for (uint32_t r0 = 0;; r0 += LEX_STAGE) {
     const bool any = r0 < wtotal;  // uniform in the wave
     if (any) {
@@ -700,7 +701,7 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
           default: lw = labs[7]; break;
         }
         const uint32_t o = k - r0;
-        stk[o] = (lw >> (8 * (j & 3))) & 0xFFu;
+        stk[o] = (uint8_t)(lw >> (8 * (j & 3)));
         sts[o] = cur;
         ste[o] = (uint32_t)(base + j + (CHUNKED ? 0 : 1));
         ++k;
