@@ -1227,7 +1227,29 @@ __device__ __forceinline__ uint32_t pk_symbol(const PArgs &a, const uint32_t *ki
 constexpr int PKK_THREADS = 1024;
 constexpr int PKK_TILES = PKK_THREADS / PK_THREADS;
 
-template <bool IDX8, bool BATCH>
+// PosView of tile `tile` (an empty view past the tile count) and the symbols of the key windows of
+// this thread's PK_IPT positions in it.
+template <bool BATCH, int KW>
+__device__ __forceinline__ void keys_window(const PArgs &a, uint64_t tile, uint64_t ntiles,
+                                            uint32_t W, PosView &pv,
+                                            uint32_t (&sym)[PK_IPT + KW - 1]) {
+  if (tile < ntiles) {
+    pv = pos_view<BATCH>(a, tile);
+  } else {
+    pv.kinds = a.kinds;
+    pv.P = 0;
+    pv.p0 = 0;
+    pv.doc = 0;
+  }
+  const int64_t lp0 = pv.p0 + (int64_t)(threadIdx.x % PK_THREADS) * PK_IPT;  // in the document
+#pragma unroll
+  for (int j = 0; j < PK_IPT + KW - 1; ++j)
+    sym[j] = j < PK_IPT + (int)W - 1 ? pk_symbol(a, pv.kinds, pv.P, lp0 - (int64_t)a.q + j) : 0u;
+}
+
+// KW: the largest key window q + k the instance handles (4 or PK_MAXW); the next iteration's
+// symbols are loaded while the current ones are processed.
+template <bool IDX8, bool BATCH, int KW>
 __global__ void __launch_bounds__(PKK_THREADS) parse_keys2(PArgs a) {
   extern __shared__ __align__(16) uint8_t pk_smem[];
   __shared__ PTup s_w[PKK_THREADS / WAVE];
@@ -1243,25 +1265,23 @@ __global__ void __launch_bounds__(PKK_THREADS) parse_keys2(PArgs a) {
   const uint64_t ntiles = BATCH ? *a.ntiles_dev : a.ntiles;
   const uint64_t nsuper = (ntiles + PKK_TILES - 1) / PKK_TILES;
   bool miss = false;
+  PosView npv;
+  uint32_t nsym[PK_IPT + KW - 1];
+  if ((uint64_t)blockIdx.x < nsuper)
+    keys_window<BATCH, KW>(a, blockIdx.x * (uint64_t)PKK_TILES + threadIdx.x / PK_THREADS, ntiles,
+                           W, npv, nsym);
   for (uint64_t sup = blockIdx.x; sup < nsuper; sup += gridDim.x) {
     const uint64_t my_tile = sup * PKK_TILES + threadIdx.x / PK_THREADS;
-    PosView pv;
-    if (my_tile < ntiles) {
-      pv = pos_view<BATCH>(a, my_tile);
-    } else {
-      pv.kinds = a.kinds;
-      pv.P = 0;
-      pv.p0 = 0;
-      pv.doc = 0;
-    }
+    const PosView pv = npv;
+    uint32_t sym[PK_IPT + KW - 1];
+#pragma unroll
+    for (int j = 0; j < PK_IPT + KW - 1; ++j) sym[j] = nsym[j];
+    if (sup + gridDim.x < nsuper)
+      keys_window<BATCH, KW>(a, (sup + gridDim.x) * PKK_TILES + threadIdx.x / PK_THREADS, ntiles,
+                             W, npv, nsym);
     const int64_t lp0 = pv.p0 + (int64_t)(threadIdx.x % PK_THREADS) * PK_IPT;  // in the document
     PTup acc = ptup_id();
     uint16_t ids[PK_IPT];
-    // the window of symbols of this thread's positions
-    uint32_t sym[PK_IPT + PK_MAXW];
-#pragma unroll
-    for (int j = 0; j < PK_IPT + PK_MAXW - 1; ++j)
-      if (j < PK_IPT + (int)W - 1) sym[j] = pk_symbol(a, pv.kinds, pv.P, lp0 - (int64_t)a.q + j);
     bool wmiss = false;
 #pragma unroll
     for (int i = 0; i < PK_IPT; ++i) {
@@ -1271,7 +1291,7 @@ __global__ void __launch_bounds__(PKK_THREADS) parse_keys2(PArgs a) {
       uint64_t di = 0;
       bool unknown = false;
 #pragma unroll
-      for (int j = 0; j < PK_MAXW; ++j) {
+      for (int j = 0; j < KW; ++j) {
         if (j < (int)W) {
           const uint32_t t = sym[i + j];
           unknown |= t >= a.nT;
@@ -2307,10 +2327,15 @@ int parse_phase1(alpacc_ctx *ctx, PArgs &a, uint64_t ntiles_max, hipStream_t st)
   const uint64_t nsuper = (ntiles_max + PKK_TILES - 1) / PKK_TILES;
   const unsigned kgrid =
       (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(nsuper, (uint64_t)std::max(1, ctx->cus) * 2));
-  if (ctx->d_index8)
-    parse_keys2<true, BATCH><<<kgrid, PKK_THREADS, ctx->index_len, st>>>(a);
+  const bool small_w = ctx->q + ctx->k <= 4;
+  if (ctx->d_index8 && small_w)
+    parse_keys2<true, BATCH, 4><<<kgrid, PKK_THREADS, ctx->index_len, st>>>(a);
+  else if (ctx->d_index8)
+    parse_keys2<true, BATCH, PK_MAXW><<<kgrid, PKK_THREADS, ctx->index_len, st>>>(a);
+  else if (small_w)
+    parse_keys2<false, BATCH, 4><<<kgrid, PKK_THREADS, 0, st>>>(a);
   else
-    parse_keys2<false, BATCH><<<kgrid, PKK_THREADS, 0, st>>>(a);
+    parse_keys2<false, BATCH, PK_MAXW><<<kgrid, PKK_THREADS, 0, st>>>(a);
   const uint64_t nb = std::max<uint64_t>(1, (ntiles_max + PSB - 1) / PSB);
   SegTup *btot = (SegTup *)ctx->buf(B_BTOT, (nb + 1) * sizeof(SegTup));
   if (!btot) {
