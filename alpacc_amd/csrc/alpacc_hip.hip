@@ -270,37 +270,61 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_reduce(LexArgs p) {
   const uint16_t *comp_tab = COMP_IN_LDS ? s_comp : p.endo_comp;
   const int lane = threadIdx.x & (WAVE - 1);
   const uint32_t wpb = blockDim.x / WAVE;
-  for (uint32_t tile = blockIdx.x * wpb + threadIdx.x / WAVE; tile < p.num_tiles;
-       tile += gridDim.x * wpb) {
-    const TileView tv = tile_view<BATCH>(p, tile);
-    const uint64_t base = (uint64_t)tv.ltile * LEX_TILE + (uint64_t)lane * LEX_IPT;
-    const int valid = base >= tv.n ? 0 : (int)min((uint64_t)LEX_IPT, tv.n - base);
-    uint32_t cls[LEX_IPT / 4];
-    load_classes(tv.ptr, tv.aligned, s_cls, base, valid, cls);
-    uint32_t v = 0;
+  const uint32_t stride = gridDim.x * wpb;
+  // two tiles per iteration: their table-lookup chains are independent and overlap
+  for (uint32_t t0 = blockIdx.x * wpb + threadIdx.x / WAVE; t0 < p.num_tiles; t0 += 2 * stride) {
+    uint32_t tile[2] = {t0, t0 + stride};
+    uint32_t v[2], total[2];
 #pragma unroll
-    for (int j = 0; j < LEX_IPT; ++j)
-      if (j < valid) v = s_step[v * C + LEX_CLS(j)];
+    for (int h = 0; h < 2; ++h) {
+      uint32_t cls[LEX_IPT / 4];
+      int valid = 0;
+      if (tile[h] < p.num_tiles) {
+        const TileView tv = tile_view<BATCH>(p, tile[h]);
+        const uint64_t base = (uint64_t)tv.ltile * LEX_TILE + (uint64_t)lane * LEX_IPT;
+        valid = base >= tv.n ? 0 : (int)min((uint64_t)LEX_IPT, tv.n - base);
+        load_classes(tv.ptr, tv.aligned, s_cls, base, valid, cls);
+      }
+      uint32_t x = 0;
+#pragma unroll
+      for (int j = 0; j < LEX_IPT; ++j)
+        if (j < valid) x = s_step[x * C + LEX_CLS(j)];
+      v[h] = x;
+    }
     // up-sweep: lane l with (l+1) % 2d == 0 holds the composition of lanes l-2d+1 .. l
 #pragma unroll
     for (int d = 1; d < WAVE; d <<= 1) {
-      const uint32_t left = __shfl(v, (lane - d) & (WAVE - 1), WAVE);
-      if (((lane + 1) & (2 * d - 1)) == 0) v = comp_lookup(comp_tab, e, left, v);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t left = __shfl(v[h], (lane - d) & (WAVE - 1), WAVE);
+        if (((lane + 1) & (2 * d - 1)) == 0) v[h] = comp_lookup(comp_tab, e, left, v[h]);
+      }
     }
-    const uint32_t total = __shfl(v, WAVE - 1, WAVE);
-    if (lane == WAVE - 1) v = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      total[h] = __shfl(v[h], WAVE - 1, WAVE);
+      if (lane == WAVE - 1) v[h] = 0;
+    }
     // down-sweep: right node r gets prefix(r) then sum(left subtree); left node gets prefix(r)
 #pragma unroll
     for (int d = WAVE / 2; d >= 1; d >>= 1) {
-      const uint32_t left = __shfl(v, (lane - d) & (WAVE - 1), WAVE);
-      const uint32_t right = __shfl(v, (lane + d) & (WAVE - 1), WAVE);
-      if (((lane + 1) & (2 * d - 1)) == 0)
-        v = comp_lookup(comp_tab, e, v, left);
-      else if (((lane + 1 + d) & (2 * d - 1)) == 0)
-        v = right;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t left = __shfl(v[h], (lane - d) & (WAVE - 1), WAVE);
+        const uint32_t right = __shfl(v[h], (lane + d) & (WAVE - 1), WAVE);
+        if (((lane + 1) & (2 * d - 1)) == 0)
+          v[h] = comp_lookup(comp_tab, e, v[h], left);
+        else if (((lane + 1 + d) & (2 * d - 1)) == 0)
+          v[h] = right;
+      }
     }
-    p.lane_excl[(uint64_t)tile * WAVE + lane] = (uint16_t)v;
-    if (lane == 0) p.tile_agg[tile] = (uint16_t)total;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (tile[h] < p.num_tiles) {
+        p.lane_excl[(uint64_t)tile[h] * WAVE + lane] = (uint16_t)v[h];
+        if (lane == 0) p.tile_agg[tile[h]] = (uint16_t)total[h];
+      }
+    }
   }
 }
 
