@@ -226,18 +226,49 @@ __device__ __forceinline__ uint32_t comp_lookup(const uint16_t *tab, uint32_t e,
 #define LEX_CLS(j) ((cls[(j) >> 2] >> (8 * ((j)&3))) & 0xFFu)
 
 // bytes of one lane -> byte classes, four per register
+__device__ __forceinline__ void classes_of_words(const uint32_t (&wds)[8], const uint8_t *s_cls,
+                                                 uint32_t (&cls)[LEX_IPT / 4]) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const uint32_t x = wds[q];
+    cls[q] = (uint32_t)s_cls[x & 0xFF] | ((uint32_t)s_cls[(x >> 8) & 0xFF] << 8) |
+             ((uint32_t)s_cls[(x >> 16) & 0xFF] << 16) | ((uint32_t)s_cls[x >> 24] << 24);
+  }
+}
+
+// The byte classes of in[base .. base + valid) (n: the document length). A document that starts
+// off a 16-byte boundary is read through the aligned 16-byte words around each lane's 32 bytes
+// (the offset is the same for all lanes of a document), as long as those words lie inside the
+// document; the rest reads bytes.
 __device__ __forceinline__ void load_classes(const uint8_t *in, bool aligned, const uint8_t *s_cls,
-                                             uint64_t base, int valid, uint32_t (&cls)[LEX_IPT / 4]) {
+                                             uint64_t base, int valid, uint32_t (&cls)[LEX_IPT / 4],
+                                             uint64_t n) {
   if (valid == LEX_IPT && aligned) {
     const uint4 a = *reinterpret_cast<const uint4 *>(in + base);
     const uint4 b = *reinterpret_cast<const uint4 *>(in + base + 16);
     const uint32_t wds[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const uint32_t x = wds[q];
-      cls[q] = (uint32_t)s_cls[x & 0xFF] | ((uint32_t)s_cls[(x >> 8) & 0xFF] << 8) |
-               ((uint32_t)s_cls[(x >> 16) & 0xFF] << 16) | ((uint32_t)s_cls[x >> 24] << 24);
+    classes_of_words(wds, s_cls, cls);
+  } else if (valid == LEX_IPT && base >= 16 && base + LEX_IPT + 16 <= n) {
+    const uint8_t *q = in + base;
+    const uint32_t off = (uint32_t)((uintptr_t)q & 15);
+    const uint4 *a4 = reinterpret_cast<const uint4 *>(q - off);  // inside [in, in + n)
+    const uint4 x0 = a4[0], x1 = a4[1], x2 = a4[2];
+    const uint32_t w[12] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w,
+                            x2.x, x2.y, x2.z, x2.w};
+    const uint32_t sh = off & 3;
+    uint32_t wds[8];
+    // word r of the window is bytes 4r .. 4r+3 from q: words off/4 + r and + r + 1, shifted
+#define LEX_ALIGN_CASE(W0)                                                              \
+  _Pragma("unroll") for (int r = 0; r < 8; ++r) wds[r] =                                \
+      __builtin_amdgcn_alignbyte(w[(W0) + r + 1], w[(W0) + r], sh);
+    switch (off >> 2) {
+      case 0: LEX_ALIGN_CASE(0) break;
+      case 1: LEX_ALIGN_CASE(1) break;
+      case 2: LEX_ALIGN_CASE(2) break;
+      default: LEX_ALIGN_CASE(3) break;
     }
+#undef LEX_ALIGN_CASE
+    classes_of_words(wds, s_cls, cls);
   } else {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -283,7 +314,7 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_reduce(LexArgs p) {
         const TileView tv = tile_view<BATCH>(p, tile[h]);
         const uint64_t base = (uint64_t)tv.ltile * LEX_TILE + (uint64_t)lane * LEX_IPT;
         valid = base >= tv.n ? 0 : (int)min((uint64_t)LEX_IPT, tv.n - base);
-        load_classes(tv.ptr, tv.aligned, s_cls, base, valid, cls);
+        load_classes(tv.ptr, tv.aligned, s_cls, base, valid, cls, tv.n);
       }
       uint32_t x = 0;
 #pragma unroll
@@ -564,7 +595,7 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   const uint64_t base = tbase + (uint64_t)lane * LEX_IPT;
   const int valid = (tile >= p.num_tiles || base >= n) ? 0 : (int)min((uint64_t)LEX_IPT, n - base);
   uint32_t cls[LEX_IPT / 4];
-  load_classes(tv.ptr, tv.aligned, s_cls, base, valid, cls);
+  load_classes(tv.ptr, tv.aligned, s_cls, base, valid, cls, tv.n);
   uint32_t s_lane = 0;
   if (tile < p.num_tiles) {
     const uint32_t s_tile = p.tile_state[tile];
@@ -1854,10 +1885,17 @@ __device__ void parents_run(uint64_t N, const Levels &lv, uint32_t *parent, cons
     const bool valid = i < N;
     uint64_t base = 0;
     if (BATCH) {
-      // the document of node i: from the document of the chunk's first node forward
-      uint32_t d = a.chunk_doc[c];
-      while (d + 1 < a.ndocs && a.pdocs[d + 1].node_off <= i && valid) ++d;
-      base = a.pdocs[d].node_off;
+      // the document of node i: the document of the chunk's first node, unless a later document
+      // starts inside the chunk (then forward, per lane)
+      const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.chunk_doc[c]);
+      const uint64_t next = d0 + 1 < a.ndocs ? a.pdocs[d0 + 1].node_off : ~0ull;
+      if (next >= (c + 1) * WAVE) {
+        base = a.pdocs[d0].node_off;
+      } else {
+        uint32_t d = d0;
+        while (d + 1 < a.ndocs && a.pdocs[d + 1].node_off <= i && valid) ++d;
+        base = a.pdocs[d].node_off;
+      }
     }
     const int32_t v = valid ? lv.L[0][i] : INT32_MAX;
     const SearchRev rv = search_rev(lv, c, lane);

@@ -132,3 +132,38 @@ def test_large_document_invariants(ctxs):
     want = _oracle(bundle, doc[: 1 << 20].tobytes())
     got = _tokens(ctx.lex_int(doc[: 1 << 20]))
     assert got == want
+
+
+@pytest.mark.parametrize("g", ["json", "sexp"])
+def test_unaligned_input(ctxs, g):
+    """Input pointers 1..15 bytes past a 16-byte boundary: lanes inside the document read the
+    aligned 16-byte words around their bytes, lanes at its edges read bytes; tokens and trees must
+    equal the C restatement's."""
+    from alpacc_amd import datagen
+    from alpacc_amd.runtime import DeviceBuffer
+    from oracle.c import coracle
+    bundle, ctx = ctxs[g]
+    orc = coracle.Oracle(bundle)
+    gen = {"json": datagen.json_doc, "sexp": datagen.sexp_doc}[g]
+    for n in (40, 2049, 70_001):
+        doc = gen(n, seed=n + 7)
+        n = len(doc)
+        want = orc.lex(doc.tobytes())
+        want = None if want is None else [tuple(int(v) for v in t) for t in zip(*want)]
+        buf = DeviceBuffer(n + 64)
+        cap = max(n, 1)
+        outs = [DeviceBuffer(4 * cap) for _ in range(3)]
+        try:
+            for off in range(1, 16):
+                buf.upload(doc, offset=off)
+                rc, cnt = ctx.lex_device(buf.ptr + off, n, outs[0].ptr, outs[1].ptr, outs[2].ptr,
+                                         cap)
+                got = None
+                if rc == 1:
+                    k, s, e = (o.download(np.uint32, cnt) for o in outs)
+                    got = [tuple(int(v) for v in t) for t in zip(k, s, e)]
+                assert got == want, (g, n, off)
+        finally:
+            buf.free()
+            for o in outs:
+                o.free()
