@@ -1594,7 +1594,16 @@ __global__ void pb_docs(PArgs a) {
 
 constexpr int PX_STAGE = 256;  // element slots per wavefront and round
 constexpr int PX_TOK = 320;    // token spans per wavefront staged in LDS (32-bit spans)
-constexpr size_t PX_TABLE_LDS_MAX = 40 << 10;  // stackw + prodw, copied to LDS per block
+
+// dynamic LDS of parse_expand2: stackw (u32, padded to 8 bytes), prodw (uint2), entry infos at a
+// 16-byte boundary
+__host__ __device__ inline size_t px_einfo_offset(size_t nstack, size_t nprod) {
+  return (4 * ((nstack + 1) & ~(size_t)1) + 8 * nprod + 15) & ~(size_t)15;
+}
+__host__ __device__ inline size_t px_lds_bytes(size_t nstack, size_t nprod, size_t E) {
+  return px_einfo_offset(nstack, nprod) + sizeof(EntInfo) * E;
+}
+constexpr size_t PX_TABLE_LDS_MAX = 32 << 10;  // stackw + prodw + entry infos, copied to LDS per block (64 KiB with the static LDS)
 
 // Per tile: the positions' prefixes (block scan), then the brackets and productions of every
 // wavefront's positions, written coalesced: each round covers a 64-aligned window of PX_STAGE
@@ -1615,9 +1624,10 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
   // the spans of the tokens the wavefront's terminal productions take (a contiguous rank range)
   constexpr int NTOK = sizeof(S) == 4 ? PX_TOK : 1;
   __shared__ uint2 s_tok[PK_THREADS / WAVE][NTOK];
-  extern __shared__ __align__(16) uint8_t px_smem[];  // stackw, then prodw
+  extern __shared__ __align__(16) uint8_t px_smem[];  // stackw, prodw, then the entry infos
   uint32_t *s_stackw = reinterpret_cast<uint32_t *>(px_smem);
   uint2 *s_prodw = reinterpret_cast<uint2 *>(px_smem + 4 * ((a.nstack + 1) & ~1u));
+  EntInfo *s_einfo = reinterpret_cast<EntInfo *>(px_smem + px_einfo_offset(a.nstack, a.nprod));
   const int lane = threadIdx.x & (WAVE - 1);
   const int w = threadIdx.x / WAVE;
   const uint64_t tile = blockIdx.x;
@@ -1640,13 +1650,15 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
   // the tables' copy into LDS overlaps the loads of the entries above
   for (uint32_t i = threadIdx.x; i < a.nstack; i += blockDim.x) s_stackw[i] = a.stackw[i];
   for (uint32_t i = threadIdx.x; i < a.nprod; i += blockDim.x) s_prodw[i] = a.prodw[i];
+  for (uint32_t i = threadIdx.x; i < a.E; i += blockDim.x) s_einfo[i] = a.einfo[i];
   const PTup pre = a.pre[tile];
+  __syncthreads();
   EntInfo inf[PK_IPT];
   PTup acc = ptup_id();
 #pragma unroll
   for (int i = 0; i < PK_IPT; ++i) {
-    inf[i] = (pbase + i < pend && ids[i] != PK_NO_ENTRY) ? a.einfo[ids[i]]
-                                                         : EntInfo{0, 0, 0, INT16_MAX, 0, 0, 0, 0};
+    const bool has = pbase + i < pend && ids[i] != PK_NO_ENTRY;
+    inf[i] = has ? s_einfo[ids[i]] : EntInfo{0, 0, 0, INT16_MAX, 0, 0, 0, 0};
     acc = ptup_op(acc, ptup_of(inf[i]));
   }
   PTup total;
@@ -2523,7 +2535,7 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, co
   ElemBufs e;
   rc = elem_bufs(ctx, B, N, e, st);
   if (rc) return rc;
-  const size_t xlds = 4 * ((ctx->nstackw + 1) & ~(size_t)1) + 8 * ctx->nprodw;
+  const size_t xlds = px_lds_bytes(ctx->nstackw, ctx->nprodw, ctx->E);
   ctx->prof_begin(S_PARSE_EXPAND, st);
   parse_expand2<S, false><<<(unsigned)ntiles, PK_THREADS, xlds, st>>>(
       a, starts, ends, e.depth, e.br, e.dl1, e.sx, e.sl1, node, span, write_nodes);
@@ -2667,7 +2679,7 @@ int run_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offset
     return ALPACC_ERR_HIP;
   }
   if (ntiles > 0) {
-    const size_t xlds = 4 * ((ctx->nstackw + 1) & ~(size_t)1) + 8 * ctx->nprodw;
+    const size_t xlds = px_lds_bytes(ctx->nstackw, ctx->nprodw, ctx->E);
     ctx->prof_begin(S_PARSE_EXPAND, st);
     parse_expand2<uint32_t, true><<<(unsigned)ntiles, PK_THREADS, xlds, st>>>(
         a, starts, ends, e.depth, e.br, e.dl1, e.sx, e.sl1, node, span, write_nodes);
@@ -2764,8 +2776,8 @@ bool build_parser_tables(alpacc_ctx *ctx, const alpacc_tables *t, std::string &e
   }
   if (stackw.empty()) stackw.push_back(0);
   if (prodw.empty()) prodw.push_back(make_uint2(0, 0));
-  if (4 * ((stackw.size() + 1) & ~(size_t)1) + 8 * prodw.size() > PX_TABLE_LDS_MAX) {
-    err = "bracket and production tables too large for LDS";
+  if (px_lds_bytes(stackw.size(), prodw.size(), E) > PX_TABLE_LDS_MAX) {
+    err = "pair entry tables too large for LDS";
     return false;
   }
   ctx->index_len = t->pair_index_len;
