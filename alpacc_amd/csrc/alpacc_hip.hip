@@ -1049,10 +1049,10 @@ __device__ __forceinline__ SearchRev search_rev(const Levels &lv, uint64_t chunk
 // then a search over the minima of the 64 chunks before that picks the chunk holding the answer,
 // and each distinct such chunk is loaded once (coalesced) and searched; a per-lane climb of the
 // min hierarchy handles the rest. All 64 lanes must call this together.
+// p0 = lift_left(v, lane, v), computed by the caller (two chunks' lifts overlap there).
 __device__ int64_t prev_le_wave(const Levels &lv, uint64_t chunk, int lane, int32_t v,
-                                const SearchRev &rv, bool need) {
+                                const SearchRev &rv, bool need, int p0) {
   const int64_t cbase = (int64_t)(chunk * WAVE);
-  const int p0 = lift_left(v, lane, v);
   int64_t res = -1;
   bool done = !need;
   if (!done && p0 > 0) {
@@ -1849,36 +1849,53 @@ __device__ void bracket_match_run(const uint16_t *br, uint64_t B, const Levels &
   const uint64_t nch = (B + WAVE - 1) / WAVE;
   const uint64_t wstride = nblk * (blockDim.x / WAVE);
   bool bad = false;
-  for (uint64_t c = blk * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch; c += wstride) {
-    const uint64_t i = c * WAVE + lane;
-    const bool valid = i < B;
-    const uint16_t b = valid ? br[i] : (uint16_t)0x8000u;
-    const int32_t v = valid ? lv.L[0][i] : INT32_MAX;
-    const SearchRev rv = search_rev(lv, c, lane);
-    const bool right = valid && !(b & 0x8000u);
-    const int64_t j = prev_le_wave(lv, c, lane, v, rv, right);
-    const int64_t jl = j - (int64_t)(c * WAVE);
-    const uint16_t bin = (uint16_t)__shfl((int)b, (int)(jl & (WAVE - 1)), WAVE);
-    bool mism = false;
-    if (right) {
-      if (j < 0) {
-        mism = true;
-      } else {
-        const uint16_t bj = (jl >= 0 && jl < WAVE) ? bin : br[j];
-        mism = (bj & 0x7FFFu) != (b & 0x7FFFu);
-      }
+  // two chunks per iteration: their in-chunk lifts (dependent permute chains) overlap
+  for (uint64_t c0 = blk * (blockDim.x / WAVE) + threadIdx.x / WAVE; c0 < nch; c0 += 2 * wstride) {
+    const uint64_t cc[2] = {c0, c0 + wstride};
+    uint16_t b[2];
+    int32_t v[2];
+    SearchRev rv[2];
+    int p0[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint64_t i = cc[h] * WAVE + lane;
+      const bool valid = cc[h] < nch && i < B;
+      b[h] = valid ? br[i] : (uint16_t)0x8000u;
+      v[h] = valid ? lv.L[0][i] : INT32_MAX;
+      rv[h] = cc[h] < nch ? search_rev(lv, cc[h], lane) : SearchRev{INT32_MAX, INT32_MAX};
     }
-    if (BATCH && mism) {
-      // the document holding bracket i: the last d with br_off <= i (offsets are monotone)
-      uint32_t lo = 0, hi = a.ndocs;
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (a.pdocs[mid].br_off <= i) lo = mid;
-        else hi = mid;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) p0[h] = lift_left(v[h], lane, v[h]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint64_t c = cc[h];
+      if (c >= nch) break;  // uniform in the wave
+      const uint64_t i = c * WAVE + lane;
+      const bool right = i < B && !(b[h] & 0x8000u);
+      const int64_t j = prev_le_wave(lv, c, lane, v[h], rv[h], right, p0[h]);
+      const int64_t jl = j - (int64_t)(c * WAVE);
+      const uint16_t bin = (uint16_t)__shfl((int)b[h], (int)(jl & (WAVE - 1)), WAVE);
+      bool mism = false;
+      if (right) {
+        if (j < 0) {
+          mism = true;
+        } else {
+          const uint16_t bj = (jl >= 0 && jl < WAVE) ? bin : br[j];
+          mism = (bj & 0x7FFFu) != (b[h] & 0x7FFFu);
+        }
       }
-      atomicOr(a.doc_flags + lo, 2u);
+      if (BATCH && mism) {
+        // the document holding bracket i: the last d with br_off <= i (offsets are monotone)
+        uint32_t lo = 0, hi = a.ndocs;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) / 2;
+          if (a.pdocs[mid].br_off <= i) lo = mid;
+          else hi = mid;
+        }
+        atomicOr(a.doc_flags + lo, 2u);
+      }
+      bad |= mism;
     }
-    bad |= mism;
   }
   if (!BATCH && __ballot(bad) != 0ull && lane == 0) atomicOr(flags, 1u);
 }
@@ -1892,6 +1909,8 @@ __device__ void parents_run(uint64_t N, const Levels &lv, uint32_t *parent, cons
   const int lane = threadIdx.x & (WAVE - 1);
   const uint64_t nch = (N + WAVE - 1) / WAVE;
   const uint64_t wstride = nblk * (blockDim.x / WAVE);
+  // one chunk per iteration: this search is bound by instruction issue, and a second chunk in
+  // flight (as in bracket_match) costs more in occupancy than it gains in overlap
   for (uint64_t c = blk * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch; c += wstride) {
     const uint64_t i = c * WAVE + lane;
     const bool valid = i < N;
@@ -1911,7 +1930,8 @@ __device__ void parents_run(uint64_t N, const Levels &lv, uint32_t *parent, cons
     }
     const int32_t v = valid ? lv.L[0][i] : INT32_MAX;
     const SearchRev rv = search_rev(lv, c, lane);
-    const int64_t j = prev_le_wave(lv, c, lane, v, rv, valid && i > base);
+    const int64_t j =
+        prev_le_wave(lv, c, lane, v, rv, valid && i > base, lift_left(v, lane, v));
     if (valid) parent[i] = i == base ? 0u : (uint32_t)((uint64_t)j - base);
   }
 }
