@@ -1189,6 +1189,9 @@ struct PArgs {
   const uint32_t *kinds;
   uint64_t T;  // tokens
   uint64_t P;  // positions: T + 2 (start and end terminal added, to_keys parser.fut:235-240)
+  // single document straight after the lexer: the token count on the device (T, P and the tile
+  // count above are then upper bounds that size the grids)
+  const uint32_t *T_dev;
   uint32_t q, k, nT, start_t, end_t, empty_t;
   const uint8_t *index8;    // dense pair index, entry or 0xFF (when E < 255)
   const uint32_t *index32;  // dense pair index, entry or 0xFFFFFFFF
@@ -1245,7 +1248,7 @@ __device__ __forceinline__ PosView pos_view(const PArgs &a, uint64_t tile) {
   PosView v;
   if (!BATCH) {
     v.kinds = a.kinds;
-    v.P = a.P;
+    v.P = a.T_dev ? std::min<uint64_t>(*a.T_dev, a.T) + 2 : a.P;
     v.p0 = (int64_t)(tile * PK_TILE);
     v.tok_base = 0;
     v.doc = 0;
@@ -1264,6 +1267,14 @@ __device__ __forceinline__ PosView pos_view(const PArgs &a, uint64_t tile) {
     v.subst = (int)doc.subst;
   }
   return v;
+}
+
+// position tiles to process: the device count in batched mode or after the lexer, else a.ntiles
+template <bool BATCH>
+__device__ __forceinline__ uint64_t pk_ntiles(const PArgs &a) {
+  if (BATCH) return *a.ntiles_dev;
+  if (a.T_dev) return (std::min<uint64_t>(*a.T_dev, a.T) + 2 + PK_TILE - 1) / PK_TILE;
+  return a.ntiles;
 }
 
 // symbol of position x of [start] ++ kinds ++ [end], the empty terminal outside (get_key,
@@ -1317,7 +1328,7 @@ __global__ void __launch_bounds__(PKK_THREADS) parse_keys2(PArgs a) {
   const int lane = threadIdx.x & (WAVE - 1);
   const int w = threadIdx.x / WAVE;
   const uint32_t W = a.q + a.k;
-  const uint64_t ntiles = BATCH ? *a.ntiles_dev : a.ntiles;
+  const uint64_t ntiles = pk_ntiles<BATCH>(a);
   const uint64_t nsuper = (ntiles + PKK_TILES - 1) / PKK_TILES;
   bool miss = false;
   PosView npv;
@@ -1467,7 +1478,7 @@ __device__ __forceinline__ SegTup seg_block_excl(SegTup v, SegTup *s_w, SegTup &
 template <bool BATCH>
 __global__ void __launch_bounds__(PSB) pscan_reduce(PArgs a, SegTup *btot) {
   __shared__ SegTup s_w[PSB / WAVE];
-  const uint64_t ntiles = BATCH ? *a.ntiles_dev : a.ntiles;
+  const uint64_t ntiles = pk_ntiles<BATCH>(a);
   const uint64_t t = (uint64_t)blockIdx.x * PSB + threadIdx.x;
   if ((uint64_t)blockIdx.x * PSB >= ntiles) return;  // whole block beyond the tiles
   const SegTup v = t < ntiles ? SegTup{a.tiles[t], tile_reset<BATCH>(a, t)} : SegTup{ptup_id(), 0};
@@ -1479,7 +1490,7 @@ __global__ void __launch_bounds__(PSB) pscan_reduce(PArgs a, SegTup *btot) {
 __global__ void __launch_bounds__(PSB) pscan_top(PArgs a, SegTup *btot, uint64_t nblocks_max) {
   __shared__ SegTup s_w[PSB / WAVE];
   __shared__ SegTup s_carry;
-  const uint64_t ntiles = a.ntiles_dev ? *a.ntiles_dev : a.ntiles;
+  const uint64_t ntiles = a.ntiles_dev ? *a.ntiles_dev : pk_ntiles<false>(a);
   const uint64_t nb = std::min<uint64_t>(nblocks_max, (ntiles + PSB - 1) / PSB);
   if (threadIdx.x == 0) s_carry = SegTup{ptup_id(), 0};
   __syncthreads();
@@ -1500,7 +1511,7 @@ __global__ void __launch_bounds__(PSB) pscan_top(PArgs a, SegTup *btot, uint64_t
 template <bool BATCH>
 __global__ void __launch_bounds__(PSB) pscan_down(PArgs a, const SegTup *bpre) {
   __shared__ SegTup s_w[PSB / WAVE];
-  const uint64_t ntiles = BATCH ? *a.ntiles_dev : a.ntiles;
+  const uint64_t ntiles = pk_ntiles<BATCH>(a);
   const uint64_t t = (uint64_t)blockIdx.x * PSB + threadIdx.x;
   if ((uint64_t)blockIdx.x * PSB >= ntiles) return;
   const SegTup v = t < ntiles ? SegTup{a.tiles[t], tile_reset<BATCH>(a, t)} : SegTup{ptup_id(), 0};
@@ -2162,9 +2173,11 @@ int build_levels(alpacc_ctx *ctx, int32_t *base, int32_t *l1, uint64_t n, Levels
 
 // The lexer. chunk == 0: unchunked (kinds/starts/ends as u32). chunk > 0: the reference's chunk
 // loop with chunks of `chunk` bytes (kinds, starts64/ends64).
+// sync = false (unchunked only): return ALPACC_ACCEPT once the kernels are queued; the verdict,
+// the token count and the time-out flag stay in the device words result[0..2] (B_SMALL + 4).
 int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, uint32_t *starts,
             uint32_t *ends, uint64_t *num_tokens, hipStream_t st, uint64_t chunk = 0,
-            uint64_t *starts64 = nullptr, uint64_t *ends64 = nullptr) {
+            uint64_t *starts64 = nullptr, uint64_t *ends64 = nullptr, bool sync = true) {
   if (!ctx->has_lexer) {
     ctx->err = "bundle has no lexer";
     return ALPACC_ERR_ARG;
@@ -2281,6 +2294,7 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   ctx->prof_end(S_LEX_EMIT, st);
   ctx->launches += 7;
   CK(hipGetLastError());
+  if (!sync) return ALPACC_ACCEPT;
   CK(hipMemcpyAsync(ctx->h_small, small + 4, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   CK(hipStreamSynchronize(st));
   if (ctx->h_small[2]) {
@@ -2496,21 +2510,26 @@ int run_search(alpacc_ctx *ctx, const PArgs &a, const ElemBufs &e, uint64_t B, u
   return 0;
 }
 
+// lexed = true: the tokens come straight from run_lex(sync = false) in this call; T is then an
+// upper bound (the input length), the key and scan kernels read the token count from the
+// lexer's device words, and the one host synchronisation below also reads the lexer's verdict.
 template <typename S>
 int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, const S *ends,
                      uint64_t T, uint32_t *parent, uint32_t *node, S *span, uint64_t cap,
-                     uint64_t *num_nodes, hipStream_t st, bool productions_only = false) {
+                     uint64_t *num_nodes, hipStream_t st, bool productions_only = false,
+                     bool lexed = false) {
   if (!ctx->has_parser) {
     ctx->err = "bundle has no parser";
     return ALPACC_ERR_NO_PARSER;
   }
   *num_nodes = 0;
-  const uint64_t P = T + 2;
-  if (P * (uint64_t)std::max(ctx->max_nb, ctx->max_np) >= (1ull << 31)) {
+  uint64_t P = T + 2;
+  const uint64_t elem_max = (uint64_t)std::max(ctx->max_nb, ctx->max_np);
+  if (!lexed && P * elem_max >= (1ull << 31)) {
     ctx->err = "token stream too long for 32-bit element counts";
     return ALPACC_ERR_ARG;
   }
-  const uint64_t ntiles = (P + PK_TILE - 1) / PK_TILE;
+  uint64_t ntiles = (P + PK_TILE - 1) / PK_TILE;
   uint32_t *small = (uint32_t *)ctx->buf(B_SMALL, 64 * sizeof(uint32_t));
   uint16_t *ent = (uint16_t *)ctx->buf(B_ENT, (ntiles * PK_TILE + 8) * sizeof(uint16_t));
   PTup *tiles = (PTup *)ctx->buf(B_PTILES, (ntiles + 1) * sizeof(PTup));
@@ -2526,6 +2545,7 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, co
   a.kinds = kinds;
   a.T = T;
   a.P = P;
+  a.T_dev = lexed ? small + 5 : nullptr;  // the lexer's result words: verdict, count, time-out
   a.ent = ent;
   a.tiles = tiles;
   a.pre = pre;
@@ -2537,11 +2557,33 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, co
   ctx->launches++;
   int rc = parse_phase1<false>(ctx, a, ntiles, st);
   if (rc) return rc;
-  CK(hipMemcpyAsync(ctx->h_small, small + 8, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  // with lexed: the lexer's words (small + 4 .. + 7) come along
+  const int sh = lexed ? 4 : 0;
+  CK(hipMemcpyAsync(ctx->h_small, small + 8 - sh, (16 + sh) * sizeof(uint32_t),
+                    hipMemcpyDeviceToHost, st));
   CK(hipStreamSynchronize(st));
-  if (ctx->h_small[0]) return ALPACC_REJECT;  // a missing pair (to_keys, parser.fut:235-240)
+  if (lexed) {
+    if (ctx->h_small[2]) {
+      ctx->err = "lexer look-back did not complete";
+      return ALPACC_ERR_TIMEOUT;
+    }
+    if (!ctx->h_small[0]) return ALPACC_REJECT;  // the lexer rejected the input
+    T = ctx->h_small[1];
+    P = T + 2;
+    if (P * elem_max >= (1ull << 31)) {
+      ctx->err = "token stream too long for 32-bit element counts";
+      return ALPACC_ERR_ARG;
+    }
+    ntiles = (P + PK_TILE - 1) / PK_TILE;
+    a.T = T;
+    a.P = P;
+    a.ntiles = ntiles;
+    a.T_dev = nullptr;  // from here on the counts are known on the host
+  }
+  const uint32_t *hs = ctx->h_small + sh;
+  if (hs[0]) return ALPACC_REJECT;  // a missing pair (to_keys, parser.fut:235-240)
   PTup tot;
-  memcpy(&tot, ctx->h_small + 8, sizeof(PTup));
+  memcpy(&tot, hs + 8, sizeof(PTup));
   // depths (parser.fut:153-166): a prefix below zero, or a non-zero end
   if (tot.nb > 0 && (tot.mind < 0 || tot.dd != 0)) return ALPACC_REJECT;
   const uint64_t B = (uint32_t)tot.nb, N = (uint32_t)tot.np;
@@ -2836,7 +2878,7 @@ alpacc_ctx *alpacc_create(const alpacc_tables *t, int device) {
   ctx->device = device;
   (void)hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device);
   std::string &err = g_create_error;
-  if (hipHostMalloc((void **)&ctx->h_small, 64, hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void **)&ctx->h_small, 256, hipHostMallocDefault) != hipSuccess) {
     err = "hipHostMalloc failed";
     alpacc_destroy(ctx);
     return nullptr;
@@ -3021,15 +3063,17 @@ int alpacc_parse(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_p
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
+  // the parser starts on the lexer's device results; the lexer's verdict and token count come
+  // back with the parser's first host synchronisation
   uint64_t T = 0;
-  int rc = run_lex(ctx, d_in, n, kinds, starts, ends, &T, st);
+  int rc = run_lex(ctx, d_in, n, kinds, starts, ends, &T, st, 0, nullptr, nullptr, false);
   if (rc != ALPACC_ACCEPT) {
     if (ctx->prof) ctx->prof_collect();
     return rc;
   }
   const int lex_launches = ctx->launches;
-  rc = run_parse_tokens<uint32_t>(ctx, kinds, starts, ends, T, d_parent, d_node, d_span, cap,
-                                  num_nodes, st);
+  rc = run_parse_tokens<uint32_t>(ctx, kinds, starts, ends, n, d_parent, d_node, d_span, cap,
+                                  num_nodes, st, false, true);
   ctx->launches += lex_launches;
   if (rc != ALPACC_ACCEPT) ctx->prof_drop(S_PARSE);
   if (ctx->prof) ctx->prof_collect();
