@@ -921,15 +921,49 @@ struct Levels {
   int nlev;
 };
 
-// minimum of every 64-element chunk: one wavefront per chunk, coalesced loads
-__global__ void min_reduce64(const int32_t *in, uint64_t n, int32_t *out, uint64_t nout) {
+// Level 2 of two min hierarchies in one launch: every wavefront reduces 64-entry groups of
+// level 1 of either hierarchy (coalesced).
+__global__ void __launch_bounds__(256) min_level2(Levels la, Levels lb) {
   const int lane = threadIdx.x & (WAVE - 1);
-  const uint64_t wstride = (uint64_t)gridDim.x * (blockDim.x / WAVE);
-  for (uint64_t c = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nout;
-       c += wstride) {
-    const uint64_t i = c * WAVE + lane;
-    const int32_t m = wave_min_i32(i < n ? in[i] : INT32_MAX);
-    if (lane == 0) out[c] = m;
+  const uint64_t na = la.nlev > 2 ? la.len[2] : 0, nb = lb.nlev > 2 ? lb.len[2] : 0;
+  const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x / WAVE);
+  for (uint64_t c = blockIdx.x * (uint64_t)(blockDim.x / WAVE) + threadIdx.x / WAVE; c < na + nb;
+       c += nw) {
+    const bool first = c < na;
+    const uint64_t k = first ? c : c - na;
+    const int32_t *src = first ? la.L[1] : lb.L[1];
+    const uint64_t slen = first ? la.len[1] : lb.len[1];
+    const uint64_t i = k * WAVE + lane;
+    const int32_t m = wave_min_i32(i < slen ? src[i] : INT32_MAX);
+    if (lane == 0) const_cast<int32_t *>(first ? la.L[2] : lb.L[2])[k] = m;
+  }
+}
+
+// Levels 3 .. of both hierarchies (small: level 3 has at most 1/4096 of level 1's entries), one
+// block, one thread per 64-entry group.
+__global__ void __launch_bounds__(1024) min_levels_top(Levels la, Levels lb) {
+  for (int h = 0; h < 2; ++h) {
+    const Levels &lv = h ? lb : la;
+    for (int l = 3; l < lv.nlev; ++l) {
+      const int32_t *src = lv.L[l - 1];
+      const uint64_t slen = lv.len[l - 1];
+      for (uint64_t k = threadIdx.x; k < lv.len[l]; k += blockDim.x) {
+        int32_t m = INT32_MAX;
+#pragma unroll
+        for (int q = 0; q < WAVE / 4; ++q) {
+          const uint64_t i = k * WAVE + 4 * q;  // levels are padded past their length
+          if (i < slen) {
+            const int4 x = *reinterpret_cast<const int4 *>(src + i);
+            m = min(m, x.x);
+            if (i + 1 < slen) m = min(m, x.y);
+            if (i + 2 < slen) m = min(m, x.z);
+            if (i + 3 < slen) m = min(m, x.w);
+          }
+        }
+        const_cast<int32_t *>(lv.L[l])[k] = m;
+      }
+      __syncthreads();  // level l before level l + 1 (one block: workgroup scope suffices)
+    }
   }
 }
 
@@ -1844,6 +1878,15 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
   }
 }
 
+// chunk minima of both element streams start at INT32_MAX (chunks shared by two wavefronts take
+// atomicMin)
+__global__ void elem_init(int32_t *dl1, uint64_t n1, int32_t *sl1, uint64_t n2) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  for (uint64_t i = t; i < n1; i += stride) dl1[i] = INT32_MAX;
+  for (uint64_t i = t; i < n2; i += stride) sl1[i] = INT32_MAX;
+}
+
 __global__ void fill_i32(int32_t *p, uint64_t n, int32_t v) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -2129,8 +2172,9 @@ T *upload(const T *h, size_t count, std::string &err) {
 // reduce-then-scan of n int2 values from `src` into `sink`; the grand total goes to *total
 // (copied to the host when total is not NULL).
 // buf_id: the scratch arena of the upper levels; hierarchies in use at the same time need their own
-int build_levels(alpacc_ctx *ctx, int32_t *base, int32_t *l1, uint64_t n, Levels &lv,
-                 hipStream_t st, int buf_id) {
+// Lays out the 64-ary min levels above level 1 (l1, written by parse_expand2) in the arena
+// buf_id; fill_levels computes them.
+int plan_levels(alpacc_ctx *ctx, int32_t *base, int32_t *l1, uint64_t n, Levels &lv, int buf_id) {
   lv.nlev = 1;
   lv.L[0] = base;
   lv.len[0] = n;
@@ -2149,7 +2193,6 @@ int build_levels(alpacc_ctx *ctx, int32_t *base, int32_t *l1, uint64_t n, Levels
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
-  int32_t *cur = l1;
   uint64_t off = 0;
   while (len > 64) {
     if (lv.nlev >= MAX_LEVELS) {
@@ -2157,15 +2200,25 @@ int build_levels(alpacc_ctx *ctx, int32_t *base, int32_t *l1, uint64_t n, Levels
       return ALPACC_ERR_ARG;
     }
     const uint64_t nl = (len + 63) / 64;
-    int32_t *next = area + off;
-    min_reduce64<<<grid_for(nl * WAVE, 256), 256, 0, st>>>(cur, len, next, nl);
-    ctx->launches++;
-    lv.L[lv.nlev] = next;
+    lv.L[lv.nlev] = area + off;
     lv.len[lv.nlev] = nl;
     lv.nlev++;
     off += ((nl + 3) & ~3ull) + 16;
-    cur = next;
     len = nl;
+  }
+  return 0;
+}
+
+// Fills the levels that plan_levels laid out, for both hierarchies: level 2 across the GPU, the
+// small levels above it in one block.
+int fill_levels(alpacc_ctx *ctx, const Levels &la, const Levels &lb, hipStream_t st) {
+  const uint64_t groups = (la.nlev > 2 ? la.len[2] : 0) + (lb.nlev > 2 ? lb.len[2] : 0);
+  if (groups == 0) return 0;
+  min_level2<<<grid_for(groups * WAVE, 256), 256, 0, st>>>(la, lb);
+  ctx->launches++;
+  if (la.nlev > 3 || lb.nlev > 3) {
+    min_levels_top<<<1, 1024, 0, st>>>(la, lb);
+    ctx->launches++;
   }
   CK(hipGetLastError());
   return 0;
@@ -2476,9 +2529,9 @@ int elem_bufs(alpacc_ctx *ctx, uint64_t B, uint64_t N, ElemBufs &e, hipStream_t 
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
-  fill_i32<<<grid_for(nbc + 16, 256, 4096), 256, 0, st>>>(e.dl1, nbc + 16, INT32_MAX);
-  fill_i32<<<grid_for(npc + 16, 256, 4096), 256, 0, st>>>(e.sl1, npc + 16, INT32_MAX);
-  ctx->launches += 2;
+  elem_init<<<grid_for(std::max(nbc, npc) + 16, 256, 4096), 256, 0, st>>>(e.dl1, nbc + 16, e.sl1,
+                                                                          npc + 16);
+  ctx->launches++;
   return 0;
 }
 
@@ -2491,8 +2544,9 @@ int run_search(alpacc_ctx *ctx, const PArgs &a, const ElemBufs &e, uint64_t B, u
   Levels lvb, lvs;
   lvb.nlev = lvs.nlev = 0;
   int rc = 0;
-  if (B > 0) rc = build_levels(ctx, e.depth, e.dl1, B, lvb, st, B_LEVELS);
-  if (!rc && N > 0) rc = build_levels(ctx, e.sx, e.sl1, N, lvs, st, B_LEVELS_S);
+  if (B > 0) rc = plan_levels(ctx, e.depth, e.dl1, B, lvb, B_LEVELS);
+  if (!rc && N > 0) rc = plan_levels(ctx, e.sx, e.sl1, N, lvs, B_LEVELS_S);
+  if (!rc) rc = fill_levels(ctx, lvb, lvs, st);
   if (rc) return rc;
   if (B > 0) {
     ctx->prof_begin(S_BRACKET_MATCH, st);
