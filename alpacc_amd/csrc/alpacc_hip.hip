@@ -317,9 +317,14 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_reduce(LexArgs p) {
         load_classes(tv.ptr, tv.aligned, s_cls, base, valid, cls, tv.n);
       }
       uint32_t x = 0;
+      if (valid == LEX_IPT) {
 #pragma unroll
-      for (int j = 0; j < LEX_IPT; ++j)
-        if (j < valid) x = s_step[x * C + LEX_CLS(j)];
+        for (int j = 0; j < LEX_IPT; ++j) x = s_step[__umul24(x, C) + LEX_CLS(j)];
+      } else {
+#pragma unroll
+        for (int j = 0; j < LEX_IPT; ++j)
+          if (j < valid) x = s_step[x * C + LEX_CLS(j)];
+      }
       v[h] = x;
     }
     // up-sweep: lane l with (l+1) % 2d == 0 holds the composition of lanes l-2d+1 .. l
@@ -613,7 +618,32 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   uint32_t labs[LEX_IPT / 4] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t s_end = s_lane;
   uint32_t lab_end = 0;
-  if (valid > 0) {
+  if (valid == LEX_IPT && base + LEX_IPT < n) {
+    // a full lane followed by another byte (every lane but a document's last): no bounds in the
+    // loop. State x class indexes fit 24 bits (the table has at most LEX_MAX_DFA entries).
+    uint32_t ent = s_dfa[__umul24(s_lane, C) + LEX_CLS(0)];
+    uint32_t s = ent & 0xFFu;
+    uint32_t lab = ent >> 9;
+    bool lab_kept = lab != ign;
+    if (((ent >> 8) & 1u) && lab_kept) startm |= 1u;
+#pragma unroll
+    for (int j = 0; j < LEX_IPT; ++j) {
+      labs[j >> 2] |= (lab & 0xFFu) << (8 * (j & 3));
+      const uint32_t cn = j + 1 < LEX_IPT ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
+      const uint32_t ent2 = s_dfa[__umul24(s, C) + cn];
+      const bool prod2 = (ent2 >> 8) & 1u;
+      if (prod2 && lab_kept) endm |= 1u << j;
+      if (j + 1 < LEX_IPT) {
+        const uint32_t lab2 = ent2 >> 9;
+        lab_kept = lab2 != ign;
+        if (prod2 && lab_kept) startm |= 1u << (j + 1);
+        s = ent2 & 0xFFu;
+        lab = lab2;
+      }
+    }
+    s_end = s;
+    lab_end = lab;
+  } else if (valid > 0) {
     uint32_t ent = s_dfa[s_lane * C + LEX_CLS(0)];
     uint32_t s = ent & 0xFFu;
     uint32_t lab = ent >> 9;
