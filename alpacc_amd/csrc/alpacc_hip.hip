@@ -1917,12 +1917,6 @@ __global__ void elem_init(int32_t *dl1, uint64_t n1, int32_t *sl1, uint64_t n2) 
   for (uint64_t i = t; i < n2; i += stride) sl1[i] = INT32_MAX;
 }
 
-__global__ void fill_i32(int32_t *p, uint64_t n, int32_t v) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    p[i] = v;
-}
-
 // brackets_matches (parser.fut:174-184): every right bracket equals (without the left bit) the
 // bracket at its previous smaller-or-equal depth. One wavefront per 64 brackets; a match inside
 // the wavefront's own chunk is read from a register.
