@@ -1887,9 +1887,12 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
         if (g >= wlo && g < whi) {
           sx[g] = svs[j];
           if (write_nodes) {
-            node[g] = nds[j];
+            // node and span are results nothing reads again here: non-temporal stores keep them
+            // from evicting the element arrays the searches read next
+            __builtin_nontemporal_store(nds[j], node + g);
             if (sizeof(S) == 4) {
-              reinterpret_cast<uint2 *>(span)[g] = make_uint2((uint32_t)v0[j], (uint32_t)v1[j]);
+              __builtin_nontemporal_store((uint64_t)(uint32_t)v0[j] | ((uint64_t)(uint32_t)v1[j] << 32),
+                                          reinterpret_cast<uint64_t *>(span) + g);
             } else {
               span[2 * (uint64_t)g] = v0[j];
               span[2 * (uint64_t)g + 1] = v1[j];
@@ -2010,7 +2013,8 @@ __device__ void parents_run(uint64_t N, const Levels &lv, uint32_t *parent, cons
     const SearchRev rv = search_rev(lv, c, lane);
     const int64_t j =
         prev_le_wave(lv, c, lane, v, rv, valid && i > base, lift_left(v, lane, v));
-    if (valid) parent[i] = i == base ? 0u : (uint32_t)((uint64_t)j - base);
+    if (valid)
+      __builtin_nontemporal_store(i == base ? 0u : (uint32_t)((uint64_t)j - base), parent + i);
   }
 }
 
