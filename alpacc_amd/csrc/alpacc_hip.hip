@@ -63,7 +63,7 @@ __device__ __forceinline__ void st_relaxed64(uint64_t *p, uint64_t v) {
 // outside its 16-lane row, or whose row the mask disables, keeps `old` (the scan identity).
 constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114,
               DPP_ROW_SHR8 = 0x118, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143,
-              DPP_WAVE_SHR1 = 0x138;
+              DPP_WAVE_SHR1 = 0x138, DPP_WAVE_SHL1 = 0x130;
 
 template <int CTRL, int ROW_MASK = 0xF>
 __device__ __forceinline__ int32_t dpp_i32(int32_t old, int32_t v) {
@@ -1249,6 +1249,34 @@ __device__ __forceinline__ PTup ptup_block_excl(PTup v, PTup *s_w, PTup &total) 
   return ptup_op(pre, ex);
 }
 
+// Adjacent cancellation. A position's trailing left brackets and the next position's leading right
+// brackets follow each other directly in the bracket stream (brackets_matches, parser.fut:174-184),
+// so the last left of the one and the first right of the other match (the previous smaller-or-equal
+// depth of that right bracket is that left bracket), then the next pair inward, and so on:
+// min(trailing lefts, leading rights) pairs. Their symbols are checked when the stream is
+// expanded and they are never written; the remaining brackets keep their depth values and their
+// matches, and the depth check still sees every bracket (the entries' totals are not changed).
+// Pairs cancel inside one wavefront's 256 positions (the neighbour comes through a DPP wave shift).
+// tlr[i]: trailing lefts | leading rights << 8 of the entry of this lane's position i (0 without
+// one); cr[i]: pairs position i cancels with position i + 1; cl0: pairs the first position cancels
+// with the previous lane's last. All lanes of the wavefront take part.
+__device__ __forceinline__ void pk_cancel(const uint32_t (&tlr)[PK_IPT], int32_t (&cr)[PK_IPT],
+                                          int32_t &cl0) {
+#pragma unroll
+  for (int i = 0; i + 1 < PK_IPT; ++i) cr[i] = (int32_t)min(tlr[i] & 0xFFu, tlr[i + 1] >> 8);
+  const uint32_t nx = (uint32_t)dpp_i32<DPP_WAVE_SHL1>(0, (int32_t)tlr[0]);  // lane 63: none
+  cr[PK_IPT - 1] = (int32_t)min(tlr[PK_IPT - 1] & 0xFFu, nx >> 8);
+  cl0 = dpp_i32<DPP_WAVE_SHR1>(0, cr[PK_IPT - 1]);  // lane 0: none
+}
+
+// brackets the cancellations above take away from this lane's positions
+__device__ __forceinline__ int32_t pk_cancelled(const int32_t (&cr)[PK_IPT], int32_t cl0) {
+  int32_t s = cl0 + cr[PK_IPT - 1];
+#pragma unroll
+  for (int i = 0; i + 1 < PK_IPT; ++i) s += 2 * cr[i];
+  return s;
+}
+
 struct PArgs {
   const uint32_t *kinds;
   uint64_t T;  // tokens
@@ -1264,6 +1292,7 @@ struct PArgs {
   uint32_t E;
   const uint32_t *stackw;  // bracket symbol (| 0x8000 left) | depth value relative to the entry << 16
   const uint2 *prodw;      // x: production | (terminal + 1) << 16; y: s relative | rank relative << 16
+  const uint16_t *ecan;    // per entry: trailing left brackets | leading right brackets << 8
   uint32_t nstack, nprod;  // lengths of stackw / prodw
   uint16_t *ent;           // [ntiles * PK_TILE] pair entry of every position
   PTup *tiles;             // [ntiles] tile totals
@@ -1384,9 +1413,13 @@ __global__ void __launch_bounds__(PKK_THREADS) parse_keys2(PArgs a) {
   extern __shared__ __align__(16) uint8_t pk_smem[];
   __shared__ PTup s_w[PKK_THREADS / WAVE];
   __shared__ EntInfo s_ei[256];
+  __shared__ uint16_t s_can[256];
   if (IDX8) {
     for (uint32_t i = threadIdx.x; i < a.index_len; i += blockDim.x) pk_smem[i] = a.index8[i];
-    for (uint32_t i = threadIdx.x; i < a.E; i += blockDim.x) s_ei[i] = a.einfo[i];
+    for (uint32_t i = threadIdx.x; i < a.E; i += blockDim.x) {
+      s_ei[i] = a.einfo[i];
+      s_can[i] = a.ecan[i];
+    }
   }
   __syncthreads();
   const int lane = threadIdx.x & (WAVE - 1);
@@ -1412,11 +1445,13 @@ __global__ void __launch_bounds__(PKK_THREADS) parse_keys2(PArgs a) {
     const int64_t lp0 = pv.p0 + (int64_t)(threadIdx.x % PK_THREADS) * PK_IPT;  // in the document
     PTup acc = ptup_id();
     uint16_t ids[PK_IPT];
+    uint32_t tlr[PK_IPT];
     bool wmiss = false;
 #pragma unroll
     for (int i = 0; i < PK_IPT; ++i) {
       const int64_t p = lp0 + i;
       ids[i] = 0;
+      tlr[i] = 0;
       if (p >= (int64_t)pv.P) continue;
       uint64_t di = 0;
       bool unknown = false;
@@ -1441,7 +1476,13 @@ __global__ void __launch_bounds__(PKK_THREADS) parse_keys2(PArgs a) {
         continue;
       }
       ids[i] = (uint16_t)id;
+      tlr[i] = IDX8 ? s_can[id] : a.ecan[id];
       acc = ptup_op(acc, ptup_of(IDX8 ? s_ei[id] : a.einfo[id]));
+    }
+    {
+      int32_t cr[PK_IPT], cl0;
+      pk_cancel(tlr, cr, cl0);
+      acc.nb -= pk_cancelled(cr, cl0);
     }
     if (BATCH) {
       if (__ballot(wmiss) != 0ull && lane == 0) atomicOr(a.doc_flags + pv.doc, 1u);
@@ -1661,7 +1702,7 @@ __global__ void pb_docs(PArgs a) {
   pd.B = (uint32_t)(inc.nb - first.nb);
   pd.N = (uint32_t)(inc.np - first.np);
   bool ok = pd.lex_ok && (a.doc_flags[d] & 1u) == 0;
-  if (pd.B > 0 && (inc.mind < 0 || inc.dd != 0)) ok = false;
+  if (inc.mind < 0 || inc.dd != 0) ok = false;  // B counts the brackets left after cancellation
   pd.status = ok ? 1u : 0u;
   pd.subst = (uint32_t)inc.nt == pd.T ? 1u : 0u;
   a.pdocs[d] = pd;
@@ -1671,12 +1712,12 @@ constexpr int PX_STAGE = 256;  // element slots per wavefront and round
 constexpr int PX_TOK = 320;    // token spans per wavefront staged in LDS (32-bit spans)
 
 // dynamic LDS of parse_expand2: stackw (u32, padded to 8 bytes), prodw (uint2), entry infos at a
-// 16-byte boundary
+// 16-byte boundary, then the entries' cancellation counts (u16)
 __host__ __device__ inline size_t px_einfo_offset(size_t nstack, size_t nprod) {
   return (4 * ((nstack + 1) & ~(size_t)1) + 8 * nprod + 15) & ~(size_t)15;
 }
 __host__ __device__ inline size_t px_lds_bytes(size_t nstack, size_t nprod, size_t E) {
-  return px_einfo_offset(nstack, nprod) + sizeof(EntInfo) * E;
+  return px_einfo_offset(nstack, nprod) + (sizeof(EntInfo) + sizeof(uint16_t)) * E;
 }
 constexpr size_t PX_TABLE_LDS_MAX = 32 << 10;  // stackw + prodw + entry infos, copied to LDS per block (64 KiB with the static LDS)
 
@@ -1703,6 +1744,7 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
   uint32_t *s_stackw = reinterpret_cast<uint32_t *>(px_smem);
   uint2 *s_prodw = reinterpret_cast<uint2 *>(px_smem + 4 * ((a.nstack + 1) & ~1u));
   EntInfo *s_einfo = reinterpret_cast<EntInfo *>(px_smem + px_einfo_offset(a.nstack, a.nprod));
+  uint16_t *s_can = reinterpret_cast<uint16_t *>(s_einfo + a.E);
   const int lane = threadIdx.x & (WAVE - 1);
   const int w = threadIdx.x / WAVE;
   const uint64_t tile = blockIdx.x;
@@ -1725,16 +1767,50 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
   // the tables' copy into LDS overlaps the loads of the entries above
   for (uint32_t i = threadIdx.x; i < a.nstack; i += blockDim.x) s_stackw[i] = a.stackw[i];
   for (uint32_t i = threadIdx.x; i < a.nprod; i += blockDim.x) s_prodw[i] = a.prodw[i];
-  for (uint32_t i = threadIdx.x; i < a.E; i += blockDim.x) s_einfo[i] = a.einfo[i];
+  for (uint32_t i = threadIdx.x; i < a.E; i += blockDim.x) {
+    s_einfo[i] = a.einfo[i];
+    s_can[i] = a.ecan[i];
+  }
   const PTup pre = a.pre[tile];
   __syncthreads();
   EntInfo inf[PK_IPT];
+  uint32_t tlr[PK_IPT];
   PTup acc = ptup_id();
 #pragma unroll
   for (int i = 0; i < PK_IPT; ++i) {
     const bool has = pbase + i < pend && ids[i] != PK_NO_ENTRY;
     inf[i] = has ? s_einfo[ids[i]] : EntInfo{0, 0, 0, INT16_MAX, 0, 0, 0, 0};
+    tlr[i] = has ? s_can[ids[i]] : 0u;
     acc = ptup_op(acc, ptup_of(inf[i]));
+  }
+  // adjacent cancellation, as in parse_keys2: the brackets every position writes, from its first
+  // remaining one (cl[i] leading rights skipped)
+  int32_t cr[PK_IPT], cl[PK_IPT], nbw[PK_IPT];
+  pk_cancel(tlr, cr, cl[0]);
+  acc.nb -= pk_cancelled(cr, cl[0]);
+#pragma unroll
+  for (int i = 0; i < PK_IPT; ++i) {
+    if (i > 0) cl[i] = cr[i - 1];
+    nbw[i] = (int32_t)inf[i].nb - cl[i] - cr[i];
+  }
+  // the cancelled pairs must match: the j-th trailing left of a position and the j-th leading
+  // right of the next one
+  {
+    uint32_t nsst[PK_IPT];  // first bracket of the next position's entry
+#pragma unroll
+    for (int i = 0; i + 1 < PK_IPT; ++i) nsst[i] = inf[i + 1].sst;
+    nsst[PK_IPT - 1] = (uint32_t)dpp_i32<DPP_WAVE_SHL1>(0, (int32_t)inf[0].sst);
+    bool mism = false;
+#pragma unroll
+    for (int i = 0; i < PK_IPT; ++i) {
+      const uint32_t last = (uint32_t)inf[i].sst + inf[i].nb - 1u;
+      for (int32_t j = 0; j < cr[i]; ++j)
+        mism |= ((s_stackw[last - (uint32_t)j] ^ s_stackw[nsst[i] + (uint32_t)j]) & 0x7FFFu) != 0u;
+    }
+    if (__ballot(mism) != 0ull && lane == 0) {
+      if (BATCH) atomicOr(a.doc_flags + pv.doc, 2u);
+      else atomicOr(a.flags, 1u);
+    }
   }
   PTup total;
   const PTup ex = ptup_block_excl(acc, s_w, total);  // also orders the table copies above
@@ -1748,13 +1824,13 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
     for (int i = 0; i < PK_IPT; ++i) {
       ob[i] = b;
       op[i] = q;
-      pos[lane * PK_IPT + i] = make_uint4((uint32_t)b, (uint32_t)d, inf[i].sst, 0u);
-      b += inf[i].nb;
+      pos[lane * PK_IPT + i] = make_uint4((uint32_t)b, (uint32_t)d, (uint32_t)(inf[i].sst + cl[i]), 0u);
+      b += nbw[i];
       q += inf[i].np;
       d += inf[i].dd;
     }
   }
-  const int32_t lb_hi = ob[PK_IPT - 1] + inf[PK_IPT - 1].nb;  // this lane's brackets: [ob[0], lb_hi)
+  const int32_t lb_hi = ob[PK_IPT - 1] + nbw[PK_IPT - 1];  // this lane's brackets: [ob[0], lb_hi)
   const int32_t lp_hi = op[PK_IPT - 1] + inf[PK_IPT - 1].np;
   // ranks of the wavefront's terminal productions: [rw_lo, rw_hi); their spans go to LDS now, so
   // that the production rounds below read no global memory
@@ -1793,7 +1869,7 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
       wave_lds_sync();
 #pragma unroll
       for (int i = 0; i < PK_IPT; ++i)
-        if (inf[i].nb && ob[i] >= r0w && ob[i] < r0w + PX_STAGE)
+        if (nbw[i] > 0 && ob[i] >= r0w && ob[i] < r0w + PX_STAGE)
           mark[ob[i] - r0w] = (uint16_t)(lane * PK_IPT + i + 1);
       wave_lds_sync();
 #pragma unroll
@@ -2082,6 +2158,7 @@ struct alpacc_ctx {
   EntInfo *d_einfo = nullptr;
   uint32_t *d_stackw = nullptr;
   uint2 *d_prodw = nullptr;
+  uint16_t *d_ecan = nullptr;  // per entry: trailing left brackets | leading right brackets << 8
   uint32_t max_nb = 0, max_np = 0;
   size_t nstackw = 0, nprodw = 0;
   // scratch (grow-only)
@@ -2506,6 +2583,7 @@ void parse_args_init(alpacc_ctx *ctx, PArgs &a) {
   a.E = ctx->E;
   a.stackw = ctx->d_stackw;
   a.prodw = ctx->d_prodw;
+  a.ecan = ctx->d_ecan;
   a.nstack = (uint32_t)ctx->nstackw;
   a.nprod = (uint32_t)ctx->nprodw;
 }
@@ -2668,7 +2746,8 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, co
   PTup tot;
   memcpy(&tot, hs + 8, sizeof(PTup));
   // depths (parser.fut:153-166): a prefix below zero, or a non-zero end
-  if (tot.nb > 0 && (tot.mind < 0 || tot.dd != 0)) return ALPACC_REJECT;
+  // (no bracket at all: mind is INT32_MAX and dd 0; cancelled brackets still count here)
+  if (tot.mind < 0 || tot.dd != 0) return ALPACC_REJECT;
   const uint64_t B = (uint32_t)tot.nb, N = (uint32_t)tot.np;
   a.subst = (!productions_only && (uint64_t)(uint32_t)tot.nt == T) ? 1 : 0;
   ctx->last_counts[0] = T;
@@ -2860,12 +2939,19 @@ bool build_parser_tables(alpacc_ctx *ctx, const alpacc_tables *t, std::string &e
   std::vector<EntInfo> info(E);
   std::vector<uint32_t> stackw;
   std::vector<uint2> prodw;
+  std::vector<uint16_t> ecan(E);
   for (uint32_t e = 0; e < E; ++e) {
     const uint32_t *e4 = t->pair_entry + 4ull * e;
     if (e4[1] < e4[0] || e4[3] < e4[2] || e4[1] > t->stacks_len || e4[3] > t->productions_len) {
       err = "pair entry out of range";
       return false;
     }
+    // the entry's leading right and trailing left brackets (at most 255 of each counted): the
+    // ones that can cancel against the neighbouring positions' brackets
+    uint32_t lr = 0, tl = 0;
+    while (lr < 255 && e4[0] + lr < e4[1] && !(t->stacks[e4[0] + lr] & 0x8000u)) ++lr;
+    while (tl < 255 && e4[1] - tl > e4[0] && (t->stacks[e4[1] - 1 - tl] & 0x8000u)) ++tl;
+    ecan[e] = (uint16_t)(tl | (lr << 8));
     EntInfo &f = info[e];
     f.nb = (uint16_t)(e4[1] - e4[0]);
     f.np = (uint16_t)(e4[3] - e4[2]);
@@ -2938,6 +3024,7 @@ bool build_parser_tables(alpacc_ctx *ctx, const alpacc_tables *t, std::string &e
   ctx->d_einfo = upload(info.data(), info.size(), err);
   ctx->d_stackw = upload(stackw.data(), stackw.size(), err);
   ctx->d_prodw = upload(prodw.data(), prodw.size(), err);
+  ctx->d_ecan = upload(ecan.data(), ecan.size(), err);
   return err.empty();
 }
 
@@ -3042,7 +3129,7 @@ void alpacc_destroy(alpacc_ctx *ctx) {
     if (s.first) (void)hipFree(s.first);
   void *tabs[] = {ctx->d_cls,    ctx->d_step,     ctx->d_comp,   ctx->d_apply,
                   ctx->d_dfa,    ctx->d_accept,   ctx->d_pair_index, ctx->d_index8,
-                  ctx->d_einfo,  ctx->d_stackw,   ctx->d_prodw};
+                  ctx->d_einfo,  ctx->d_stackw,   ctx->d_prodw,  ctx->d_ecan};
   for (void *p : tabs)
     if (p) (void)hipFree(p);
   if (ctx->h_small) (void)hipHostFree(ctx->h_small);

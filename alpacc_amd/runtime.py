@@ -427,6 +427,44 @@ class Context:
         return (pb.download(np.uint32, nn, self.stream), nb.download(np.uint32, nn, self.stream),
                 spb.download(np.uint32, 2 * nn, self.stream).reshape(-1, 2))
 
+    def parse_tokens(self, kinds, starts, ends, cap: int | None = None):
+        """``parser.parse_int`` (parser.fut:334-336) on an existing token stream: None when
+        rejected, else (parent, node, span[N,2]) as ``parse_int`` returns them."""
+        kinds = np.ascontiguousarray(kinds, dtype=np.uint32)
+        starts = np.ascontiguousarray(starts, dtype=np.uint32)
+        ends = np.ascontiguousarray(ends, dtype=np.uint32)
+        t = int(kinds.shape[0])
+        if starts.shape[0] != t or ends.shape[0] != t:
+            raise ValueError("kinds, starts and ends differ in length")
+        toks = self.buffer("tokens", 12 * t + 12)
+        for i, arr in enumerate((kinds, starts, ends)):
+            if t:
+                toks.upload(arr, self.stream, offset=4 * t * i)
+        if cap is None:
+            cap = max(16, 8 * t + 16)
+        while True:
+            pb = self.buffer("parent", 4 * cap)
+            nb = self.buffer("node", 4 * cap)
+            spb = self.buffer("span", 8 * cap)
+            nn = ctypes.c_uint64(0)
+            rc = self.lib.alpacc_parse_tokens(self.handle, ctypes.c_void_p(toks.ptr),
+                                              ctypes.c_void_p(toks.ptr + 4 * t),
+                                              ctypes.c_void_p(toks.ptr + 8 * t), t,
+                                              ctypes.c_void_p(pb.ptr), ctypes.c_void_p(nb.ptr),
+                                              ctypes.c_void_p(spb.ptr), cap, ctypes.byref(nn),
+                                              self.stream)
+            if rc == -3:
+                cap = nn.value
+                continue
+            if rc < 0:
+                raise self._err(rc)
+            break
+        if rc != ACCEPT:
+            return None
+        nn = nn.value
+        return (pb.download(np.uint32, nn, self.stream), nb.download(np.uint32, nn, self.stream),
+                spb.download(np.uint32, 2 * nn, self.stream).reshape(-1, 2))
+
     def _parse_int_chunked(self, data, cap, chunk: int):
         d_in, n = self.upload_doc(data)
         if cap is None:
