@@ -169,7 +169,8 @@ void alpacc_free(void *ptr);
 int alpacc_last_launch_count(alpacc_ctx *ctx);
 
 /* Element counts of the last parser call on `ctx`: counts[0] tokens, [1] key positions (T + 2;
- * tile-padded in batched calls), [2] brackets, [3] productions (tree nodes). */
+ * tile-padded in batched calls), [2] brackets written for the bracket search (those left after
+ * neighbouring positions' matching brackets cancel), [3] productions (tree nodes). */
 int alpacc_last_counts(alpacc_ctx *ctx, uint64_t *counts);
 
 /* Device-time profiling: when enabled, every call records HIP events on its stream around each
