@@ -8,7 +8,13 @@ and serves a stream of documents:
   pinned staging buffers;
 * a worker that becomes free takes the requests waiting at that moment, in arrival order, as long
   as their estimated HBM footprint fits the shared budget (admission by footprint) and the batch
-  limits hold, and runs them as one ``alpacc_parse_batch`` call;
+  limits hold, and runs them as one ``alpacc_parse_batch`` call. A batch holds requests of one
+  size class (below ``large_bytes``, or from it up): small requests are not held up by a large
+  one's copies;
+* with ``warmup`` (a document as large as the largest batch), every worker runs it once before
+  serving, so that its device arenas, its pinned staging and its output buffers already have their
+  final sizes: growing them while serving frees device memory, and a free waits for every stream
+  of the device;
 * requests that arrive while batches run wait for the next free worker, so several batches overlap
   on the device (one per worker stream) and arrivals are coalesced into the next launch;
 * each request gets exactly what ``parse_int`` gives it alone: the batched entry processes every
@@ -50,6 +56,11 @@ class Request:
     tree: tuple | None = None       # (parent, node, span[N, 2]) when accepted
     t_done: float = 0.0
     batch: int = -1                 # which launch served it
+    # when its batch was taken, staged in pinned memory, parsed on the device (copy in included)
+    # and copied back (perf_counter seconds)
+    t_taken: float = 0.0
+    t_staged: float = 0.0
+    t_parsed: float = 0.0
     done: threading.Event = field(default_factory=threading.Event)
 
     @property
@@ -61,11 +72,16 @@ class Server:
     """Continuous batching of parse requests on one GPU."""
 
     def __init__(self, bundle, workers: int = 4, hbm_budget: float = 200e9,
-                 max_batch_docs: int = 4096, max_batch_bytes: int = 256 << 20, device: int = 0):
+                 max_batch_docs: int = 4096, max_batch_bytes: int = 256 << 20, device: int = 0,
+                 large_bytes: int = 4 << 20, warmup=None):
         self.bundle = bundle
         self.budget = int(hbm_budget)
         self.max_docs = max_batch_docs
         self.max_bytes = max_batch_bytes
+        self.large = large_bytes
+        self._warm = None if warmup is None else np.ascontiguousarray(
+            np.frombuffer(bytes(warmup), dtype=np.uint8) if isinstance(warmup, (bytes, bytearray))
+            else warmup, dtype=np.uint8)
         self.device = device
         self._cv = threading.Condition()
         self._pending: deque = deque()
@@ -106,11 +122,16 @@ class Server:
 
     # -- worker side ----------------------------------------------------------------------------
     def _admit(self) -> list:
-        """Requests waiting now, in arrival order, while the footprint and batch limits hold."""
-        batch, nbytes = [], 0
+        """Requests waiting now, in arrival order, of the first one's size class, while the
+        footprint and batch limits hold."""
+        batch, nbytes, skipped = [], 0, []
+        large = None
         while self._pending and len(batch) < self.max_docs:
             r = self._pending[0]
             n = r.data.shape[0]
+            if large is not None and (n >= self.large) != large:
+                skipped.append(self._pending.popleft())  # the other class: left for later
+                continue
             fp = footprint(n)
             if batch and (nbytes + n > self.max_bytes or self._inflight + fp > self.budget):
                 break
@@ -118,8 +139,10 @@ class Server:
                 break  # wait for memory to free up
             self._pending.popleft()
             batch.append(r)
+            large = n >= self.large
             nbytes += n
             self._inflight += fp
+        self._pending.extendleft(reversed(skipped))  # back in arrival order
         return batch
 
     def _worker(self, w: int):
@@ -130,6 +153,8 @@ class Server:
             stream = lib.alpacc_stream_create()
             ctx.stream = stream
             state = {"in": None, "pin_in": None, "out": None, "ratio": 1.0}
+            if self._warm is not None:
+                self._run(ctx, lib, stream, state, [Request(-1, self._warm)], -1)
         except Exception as exc:  # reported to the constructor
             self._errors.append(exc)
             self._ready.wait()
@@ -189,6 +214,7 @@ class Server:
         return state[key]
 
     def _run(self, ctx, lib, stream, state, batch, launch):
+        t_taken = time.perf_counter()
         lengths = np.array([r.data.shape[0] for r in batch], dtype=np.uint64)
         offsets = np.zeros(len(batch), dtype=np.uint64)
         if len(batch) > 1:
@@ -197,6 +223,7 @@ class Server:
         pin = self._grow_pin(state, "pin_in", total, lib)
         for r, o in zip(batch, offsets.tolist()):
             pin.array[o:o + r.data.shape[0]] = r.data
+        t_staged = time.perf_counter()
         (d_in,) = self._grow_dev(state, "in", [total], lib)
         if total and lib.alpacc_copy_to_device(d_in.ptr, pin.ptr, total, stream) != 0:
             raise RuntimeError("host to device copy failed")
@@ -209,6 +236,7 @@ class Server:
                 cap = int(nodes * 1.25) + 16
                 continue
             break
+        t_parsed = time.perf_counter()
         if total:
             state["ratio"] = max(state["ratio"] * 0.9, nodes / total)
         # the batch's trees come back into fresh arrays; every request gets views of its range
@@ -226,6 +254,7 @@ class Server:
                 o, c = int(x["offset"]), int(x["count"])
                 r.tree = (par[o:o + c], nd[o:o + c], sp[o:o + c])
             r.batch = launch
+            r.t_taken, r.t_staged, r.t_parsed = t_taken, t_staged, t_parsed
             r.t_done = now
             r.done.set()
 
@@ -278,8 +307,17 @@ def latency_stats(reqs) -> dict:
     lat = np.array([r.latency for r in reqs]) * 1e3
     span = max(r.t_done for r in reqs) - min(r.arrival for r in reqs)
     nbytes = sum(int(r.data.shape[0]) for r in reqs)
-    return {"p50_ms": round(float(np.percentile(lat, 50)), 3),
-            "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "mean_ms": round(float(lat.mean()), 3),
-            "GBps": round(nbytes / span / 1e9, 3), "requests": len(reqs), "bytes": nbytes,
-            "seconds": round(span, 3)}
+    out = {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+           "p99_ms": round(float(np.percentile(lat, 99)), 3),
+           "mean_ms": round(float(lat.mean()), 3),
+           "GBps": round(nbytes / span / 1e9, 3), "requests": len(reqs), "bytes": nbytes,
+           "seconds": round(span, 3)}
+    # where the slowest 1 % of requests spent their time (continuous batching only)
+    tail = [r for r, x in zip(reqs, lat) if x >= np.percentile(lat, 99) and r.t_taken > 0]
+    if tail:
+        ph = np.array([[r.t_taken - r.arrival, r.t_staged - r.t_taken, r.t_parsed - r.t_staged,
+                        r.t_done - r.t_parsed, r.data.shape[0] / 2**20] for r in tail])
+        out["p99_tail"] = dict(zip(("wait_ms", "stage_ms", "parse_ms", "copy_back_ms", "MiB"),
+                                   (round(float(x), 2) for x in ph.mean(axis=0) *
+                                    np.array([1e3, 1e3, 1e3, 1e3, 1.0]))))
+    return out

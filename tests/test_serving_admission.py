@@ -1,18 +1,19 @@
 """Admission by HBM footprint in the serving loop (pure host logic, no GPU): requests are taken in
 arrival order while their footprint fits the budget and the batch limits; a request that does not
-fit waits until memory frees up."""
+fit waits until memory frees up; a batch holds one size class."""
 import threading
 from collections import deque
 
 import numpy as np
 
 
-def _server(budget, max_docs=100, max_bytes=1 << 30):
+def _server(budget, max_docs=100, max_bytes=1 << 30, large=1 << 40):
     from alpacc_amd import serving
     s = serving.Server.__new__(serving.Server)
     s.budget = budget
     s.max_docs = max_docs
     s.max_bytes = max_bytes
+    s.large = large
     s._pending = deque()
     s._inflight = 0
     s._cv = threading.Condition()
@@ -46,6 +47,22 @@ def test_batch_limits():
     for i in range(5):
         s2._pending.append(_req(i, 10))
     assert [r.id for r in s2._admit()] == [0, 1, 2]
+
+
+def test_size_classes():
+    """Small requests batch without the large ones between them, and the large ones keep their
+    order for the next batch."""
+    s = _server(budget=1 << 40, large=1000)
+    for i, n in enumerate((10, 5000, 20, 3000, 30)):
+        s._pending.append(_req(i, n))
+    assert [r.id for r in s._admit()] == [0, 2, 4]
+    assert [r.id for r in s._pending] == [1, 3]
+    assert [r.id for r in s._admit()] == [1, 3]
+    s2 = _server(budget=1 << 40, large=1000, max_bytes=6000)
+    for i, n in enumerate((5000, 10, 3000, 20)):
+        s2._pending.append(_req(i, n))
+    assert [r.id for r in s2._admit()] == [0]   # 3000 more would pass max_bytes
+    assert [r.id for r in s2._pending] == [1, 2, 3]
 
 
 def test_workload_shapes():
