@@ -1395,10 +1395,7 @@ __device__ __forceinline__ void keys_window(const PArgs &a, uint64_t tile, uint6
   if (tile < ntiles) {
     pv = pos_view<BATCH>(a, tile);
   } else {
-    pv.kinds = a.kinds;
-    pv.P = 0;
-    pv.p0 = 0;
-    pv.doc = 0;
+    pv = PosView{a.kinds, 0, 0, 0, 0, false, 0};  // every field set: the view stays in registers
   }
   const int64_t lp0 = pv.p0 + (int64_t)(threadIdx.x % PK_THREADS) * PK_IPT;  // in the document
 #pragma unroll
