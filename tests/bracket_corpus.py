@@ -46,8 +46,9 @@ def token_kinds(bundle) -> list:
 
 
 def short_streams(bundle, max_len: int):
-    """Every token kind stream of length 1 .. max_len."""
+    """Every token kind stream of length 0 .. max_len."""
     ks = token_kinds(bundle)
+    yield np.zeros(0, dtype=np.uint32)
     for n in range(1, max_len + 1):
         for idx in np.ndindex(*([len(ks)] * n)):
             yield np.array([ks[i] for i in idx], dtype=np.uint32)
