@@ -1257,15 +1257,15 @@ __device__ __forceinline__ PTup ptup_block_excl(PTup v, PTup *s_w, PTup &total) 
 // expanded and they are never written; the remaining brackets keep their depth values and their
 // matches, and the depth check still sees every bracket (the entries' totals are not changed).
 // Pairs cancel inside one wavefront's 256 positions (the neighbour comes through a DPP wave shift).
-// tlr[i]: trailing lefts | leading rights << 8 of the entry of this lane's position i (0 without
-// one); cr[i]: pairs position i cancels with position i + 1; cl0: pairs the first position cancels
+// tlr[i]: trailing lefts | leading rights << 8 (| brackets between << 16) of the entry of this
+// lane's position i (0 without one); cr[i]: pairs position i cancels with position i + 1; cl0: pairs the first position cancels
 // with the previous lane's last. All lanes of the wavefront take part.
 __device__ __forceinline__ void pk_cancel(const uint32_t (&tlr)[PK_IPT], int32_t (&cr)[PK_IPT],
                                           int32_t &cl0) {
 #pragma unroll
-  for (int i = 0; i + 1 < PK_IPT; ++i) cr[i] = (int32_t)min(tlr[i] & 0xFFu, tlr[i + 1] >> 8);
+  for (int i = 0; i + 1 < PK_IPT; ++i) cr[i] = (int32_t)min(tlr[i] & 0xFFu, (tlr[i + 1] >> 8) & 0xFFu);
   const uint32_t nx = (uint32_t)dpp_i32<DPP_WAVE_SHL1>(0, (int32_t)tlr[0]);  // lane 63: none
-  cr[PK_IPT - 1] = (int32_t)min(tlr[PK_IPT - 1] & 0xFFu, nx >> 8);
+  cr[PK_IPT - 1] = (int32_t)min(tlr[PK_IPT - 1] & 0xFFu, (nx >> 8) & 0xFFu);
   cl0 = dpp_i32<DPP_WAVE_SHR1>(0, cr[PK_IPT - 1]);  // lane 0: none
 }
 
@@ -1275,6 +1275,75 @@ __device__ __forceinline__ int32_t pk_cancelled(const int32_t (&cr)[PK_IPT], int
 #pragma unroll
   for (int i = 0; i + 1 < PK_IPT; ++i) s += 2 * cr[i];
   return s;
+}
+
+// Second level. After the adjacent pairs are gone, a position's remaining trailing lefts and the
+// remaining leading rights of the next position that still has brackets (the positions between
+// have none left) follow each other directly as well: min(lefts, rights) more pairs. Inside the
+// wavefront: the next such position in another lane comes through a ballot and a permute.
+// Per position: n1 brackets left after the adjacent cancellation, r1 leading rights and l1
+// trailing lefts among them, f1 the stackw index of its first remaining bracket (expand only).
+// Out: cr2[i] pairs position i cancels with its next position that has brackets left, nf[i] that
+// position's f1; with WANT_CL also cl2[i], the pairs position i cancels with its previous one.
+// All lanes of the wavefront take part.
+template <bool WANT_CL>
+__device__ __forceinline__ void pk_cancel2(const int32_t (&n1)[PK_IPT],
+                                           const int32_t (&r1)[PK_IPT],
+                                           const int32_t (&l1)[PK_IPT],
+                                           const uint32_t (&f1)[PK_IPT], int lane,
+                                           int32_t (&cr2)[PK_IPT], uint32_t (&nf)[PK_IPT],
+                                           int32_t (&cl2)[PK_IPT]) {
+  // the lane's first position with brackets left
+  int32_t fr = 0;
+  uint32_t ff = 0;
+  bool any = false;
+#pragma unroll
+  for (int i = PK_IPT - 1; i >= 0; --i) {
+    if (n1[i] > 0) {
+      fr = r1[i];
+      ff = f1[i];
+      any = true;
+    }
+  }
+  const uint64_t mask = __ballot(any);
+  const uint64_t above = lane == WAVE - 1 ? 0ull : (mask >> (lane + 1));
+  const int nl = above ? lane + 1 + (int)__builtin_ctzll(above) : lane;
+  const int32_t xr_all = __shfl(fr, nl, WAVE);
+  const uint32_t xf_all = (uint32_t)__shfl((int)ff, nl, WAVE);
+  const int32_t xr = above ? xr_all : 0;
+#pragma unroll
+  for (int i = 0; i < PK_IPT; ++i) {
+    int32_t nr = xr;  // leading rights left at the next position with brackets
+    uint32_t nfi = xf_all;
+#pragma unroll
+    for (int k = PK_IPT - 1; k > i; --k) {
+      if (n1[k] > 0) {
+        nr = r1[k];
+        nfi = f1[k];
+      }
+    }
+    cr2[i] = min(l1[i], nr);
+    nf[i] = nfi;
+  }
+  if (WANT_CL) {
+    // the lane's last position with brackets left: the pairs it cancels forward
+    int32_t lc = 0;
+#pragma unroll
+    for (int i = 0; i < PK_IPT; ++i)
+      if (n1[i] > 0) lc = cr2[i];
+    const uint64_t below = mask & ((1ull << lane) - 1ull);
+    const int pl = below ? 63 - (int)__builtin_clzll(below) : lane;
+    const int32_t xc_all = __shfl(lc, pl, WAVE);
+    const int32_t xc = below ? xc_all : 0;
+#pragma unroll
+    for (int k = 0; k < PK_IPT; ++k) {
+      int32_t pc = xc;
+#pragma unroll
+      for (int i = 0; i < k; ++i)
+        if (n1[i] > 0) pc = cr2[i];
+      cl2[k] = n1[k] > 0 ? pc : 0;
+    }
+  }
 }
 
 struct PArgs {
@@ -1292,7 +1361,7 @@ struct PArgs {
   uint32_t E;
   const uint32_t *stackw;  // bracket symbol (| 0x8000 left) | depth value relative to the entry << 16
   const uint2 *prodw;      // x: production | (terminal + 1) << 16; y: s relative | rank relative << 16
-  const uint16_t *ecan;    // per entry: trailing left brackets | leading right brackets << 8
+  const uint32_t *ecan;    // per entry: trailing lefts | leading rights << 8 | brackets between << 16
   uint32_t nstack, nprod;  // lengths of stackw / prodw
   uint16_t *ent;           // [ntiles * PK_TILE] pair entry of every position
   PTup *tiles;             // [ntiles] tile totals
@@ -1406,11 +1475,14 @@ __device__ __forceinline__ void keys_window(const PArgs &a, uint64_t tile, uint6
 // KW: the largest key window q + k the instance handles (4 or PK_MAXW); the next iteration's
 // symbols are loaded while the current ones are processed.
 template <bool IDX8, bool BATCH, int KW>
-__global__ void __launch_bounds__(PKK_THREADS) parse_keys2(PArgs a) {
+// Key windows of at most 4 terminals: two 1024-thread blocks per CU, so 8 waves per SIMD (at most
+// 64 VGPRs).
+__global__ void __launch_bounds__(PKK_THREADS) __attribute__((amdgpu_waves_per_eu(KW == 4 ? 8 : 1)))
+parse_keys2(PArgs a) {
   extern __shared__ __align__(16) uint8_t pk_smem[];
   __shared__ PTup s_w[PKK_THREADS / WAVE];
   __shared__ EntInfo s_ei[256];
-  __shared__ uint16_t s_can[256];
+  __shared__ uint32_t s_can[256];
   if (IDX8) {
     for (uint32_t i = threadIdx.x; i < a.index_len; i += blockDim.x) pk_smem[i] = a.index8[i];
     for (uint32_t i = threadIdx.x; i < a.E; i += blockDim.x) {
@@ -1441,13 +1513,12 @@ __global__ void __launch_bounds__(PKK_THREADS) parse_keys2(PArgs a) {
                              W, npv, nsym);
     const int64_t lp0 = pv.p0 + (int64_t)(threadIdx.x % PK_THREADS) * PK_IPT;  // in the document
     PTup acc = ptup_id();
-    uint16_t ids[PK_IPT];
+    uint32_t idp[PK_IPT / 2] = {0u, 0u};  // entry of every position, two u16 to a word
     uint32_t tlr[PK_IPT];
     bool wmiss = false;
 #pragma unroll
     for (int i = 0; i < PK_IPT; ++i) {
       const int64_t p = lp0 + i;
-      ids[i] = 0;
       tlr[i] = 0;
       if (p >= (int64_t)pv.P) continue;
       uint64_t di = 0;
@@ -1469,10 +1540,10 @@ __global__ void __launch_bounds__(PKK_THREADS) parse_keys2(PArgs a) {
       }
       if (id == 0xFFFFFFFFu) {
         wmiss = true;
-        ids[i] = PK_NO_ENTRY;  // expands to nothing: the document is rejected anyway
+        idp[i >> 1] |= (uint32_t)PK_NO_ENTRY << (16 * (i & 1));  // expands to nothing: rejected anyway
         continue;
       }
-      ids[i] = (uint16_t)id;
+      idp[i >> 1] |= id << (16 * (i & 1));
       tlr[i] = IDX8 ? s_can[id] : a.ecan[id];
       acc = ptup_op(acc, ptup_of(IDX8 ? s_ei[id] : a.einfo[id]));
     }
@@ -1480,6 +1551,35 @@ __global__ void __launch_bounds__(PKK_THREADS) parse_keys2(PArgs a) {
       int32_t cr[PK_IPT], cl0;
       pk_cancel(tlr, cr, cl0);
       acc.nb -= pk_cancelled(cr, cl0);
+      // second level (pk_cancel2; only the pair count is needed here): per position the leading
+      // rights and trailing lefts left after the first level, and whether any bracket is left,
+      // packed into one word so that the tile's registers stay within 8 waves per SIMD
+      uint32_t lv[PK_IPT];
+#pragma unroll
+      for (int i = 0; i < PK_IPT; ++i) {
+        const int32_t c_l = i > 0 ? cr[i - 1] : cl0;
+        const uint32_t r1 = ((tlr[i] >> 8) & 0xFFu) - (uint32_t)c_l;
+        const uint32_t l1 = (tlr[i] & 0xFFu) - (uint32_t)cr[i];
+        lv[i] = r1 | (l1 << 8) | ((r1 | l1 | (tlr[i] >> 16)) != 0u ? 0x10000u : 0u);
+      }
+      uint32_t fr = 0;
+#pragma unroll
+      for (int i = PK_IPT - 1; i >= 0; --i)
+        if (lv[i] >> 16) fr = lv[i] & 0xFFu;
+      const uint64_t mask = __ballot((lv[0] | lv[1] | lv[2] | lv[3]) >> 16);
+      const uint64_t above = lane == WAVE - 1 ? 0ull : (mask >> (lane + 1));
+      const int nl = above ? lane + 1 + (int)__builtin_ctzll(above) : lane;
+      const uint32_t xr_all = (uint32_t)__shfl((int)fr, nl, WAVE);
+      int32_t c2 = 0;
+#pragma unroll
+      for (int i = 0; i < PK_IPT; ++i) {
+        uint32_t nr = above ? xr_all : 0u;
+#pragma unroll
+        for (int k = PK_IPT - 1; k > i; --k)
+          if (lv[k] >> 16) nr = lv[k] & 0xFFu;
+        c2 += (int32_t)min((lv[i] >> 8) & 0xFFu, nr);
+      }
+      acc.nb -= 2 * c2;
     }
     if (BATCH) {
       if (__ballot(wmiss) != 0ull && lane == 0) atomicOr(a.doc_flags + pv.doc, 1u);
@@ -1491,12 +1591,12 @@ __global__ void __launch_bounds__(PKK_THREADS) parse_keys2(PArgs a) {
     const uint64_t pend = my_tile * PK_TILE + (pv.P > (uint64_t)pv.p0 ? pv.P - (uint64_t)pv.p0 : 0);
     if (pbase + PK_IPT <= pend) {
       uint2 packed;
-      packed.x = (uint32_t)ids[0] | ((uint32_t)ids[1] << 16);
-      packed.y = (uint32_t)ids[2] | ((uint32_t)ids[3] << 16);
+      packed.x = idp[0];
+      packed.y = idp[1];
       *reinterpret_cast<uint2 *>(a.ent + pbase) = packed;
     } else {
       for (int i = 0; i < PK_IPT; ++i)
-        if (pbase + i < pend) a.ent[pbase + i] = ids[i];
+        if (pbase + i < pend) a.ent[pbase + i] = (uint16_t)(idp[i >> 1] >> (16 * (i & 1)));
     }
     const PTup incl = ptup_wave_incl(acc, lane);
     if (lane == WAVE - 1) s_w[w] = incl;
@@ -1709,12 +1809,12 @@ constexpr int PX_STAGE = 256;  // element slots per wavefront and round
 constexpr int PX_TOK = 320;    // token spans per wavefront staged in LDS (32-bit spans)
 
 // dynamic LDS of parse_expand2: stackw (u32, padded to 8 bytes), prodw (uint2), entry infos at a
-// 16-byte boundary, then the entries' cancellation counts (u16)
+// 16-byte boundary, then the entries' cancellation counts (u32)
 __host__ __device__ inline size_t px_einfo_offset(size_t nstack, size_t nprod) {
   return (4 * ((nstack + 1) & ~(size_t)1) + 8 * nprod + 15) & ~(size_t)15;
 }
 __host__ __device__ inline size_t px_lds_bytes(size_t nstack, size_t nprod, size_t E) {
-  return px_einfo_offset(nstack, nprod) + (sizeof(EntInfo) + sizeof(uint16_t)) * E;
+  return px_einfo_offset(nstack, nprod) + (sizeof(EntInfo) + sizeof(uint32_t)) * E;
 }
 constexpr size_t PX_TABLE_LDS_MAX = 32 << 10;  // stackw + prodw + entry infos, copied to LDS per block (64 KiB with the static LDS)
 
@@ -1741,7 +1841,7 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
   uint32_t *s_stackw = reinterpret_cast<uint32_t *>(px_smem);
   uint2 *s_prodw = reinterpret_cast<uint2 *>(px_smem + 4 * ((a.nstack + 1) & ~1u));
   EntInfo *s_einfo = reinterpret_cast<EntInfo *>(px_smem + px_einfo_offset(a.nstack, a.nprod));
-  uint16_t *s_can = reinterpret_cast<uint16_t *>(s_einfo + a.E);
+  uint32_t *s_can = reinterpret_cast<uint32_t *>(s_einfo + a.E);
   const int lane = threadIdx.x & (WAVE - 1);
   const int w = threadIdx.x / WAVE;
   const uint64_t tile = blockIdx.x;
@@ -1780,18 +1880,33 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
     tlr[i] = has ? s_can[ids[i]] : 0u;
     acc = ptup_op(acc, ptup_of(inf[i]));
   }
-  // adjacent cancellation, as in parse_keys2: the brackets every position writes, from its first
-  // remaining one (cl[i] leading rights skipped)
-  int32_t cr[PK_IPT], cl[PK_IPT], nbw[PK_IPT];
+  // both levels of cancellation, as in parse_keys2: the brackets every position writes, from its
+  // first remaining one (skip[i] leading rights skipped)
+  int32_t cr[PK_IPT], cl[PK_IPT], nbw[PK_IPT], skip[PK_IPT];
+  int32_t r1[PK_IPT], l1[PK_IPT], cr2[PK_IPT], cl2[PK_IPT];
+  uint32_t f1[PK_IPT], nf[PK_IPT];
   pk_cancel(tlr, cr, cl[0]);
   acc.nb -= pk_cancelled(cr, cl[0]);
 #pragma unroll
   for (int i = 0; i < PK_IPT; ++i) {
     if (i > 0) cl[i] = cr[i - 1];
     nbw[i] = (int32_t)inf[i].nb - cl[i] - cr[i];
+    r1[i] = (int32_t)((tlr[i] >> 8) & 0xFFu) - cl[i];
+    l1[i] = (int32_t)(tlr[i] & 0xFFu) - cr[i];
+    f1[i] = (uint32_t)inf[i].sst + (uint32_t)cl[i];
+  }
+  pk_cancel2<true>(nbw, r1, l1, f1, lane, cr2, nf, cl2);
+#pragma unroll
+  for (int i = 0; i < PK_IPT; ++i) {
+    // the lane's own brackets (its positions' offsets come from a scan over the lanes; the tile
+    // total equals parse_keys2's, which counts both sides of a pair where the left one sits)
+    acc.nb -= cl2[i] + cr2[i];
+    nbw[i] -= cl2[i] + cr2[i];
+    skip[i] = cl[i] + cl2[i];
   }
   // the cancelled pairs must match: the j-th trailing left of a position and the j-th leading
-  // right of the next one
+  // right of the next one; at the second level, the j-th remaining trailing left and the j-th
+  // remaining leading right of the next position with brackets left
   {
     uint32_t nsst[PK_IPT];  // first bracket of the next position's entry
 #pragma unroll
@@ -1803,6 +1918,9 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
       const uint32_t last = (uint32_t)inf[i].sst + inf[i].nb - 1u;
       for (int32_t j = 0; j < cr[i]; ++j)
         mism |= ((s_stackw[last - (uint32_t)j] ^ s_stackw[nsst[i] + (uint32_t)j]) & 0x7FFFu) != 0u;
+      const uint32_t last2 = last - (uint32_t)cr[i];
+      for (int32_t j = 0; j < cr2[i]; ++j)
+        mism |= ((s_stackw[last2 - (uint32_t)j] ^ s_stackw[nf[i] + (uint32_t)j]) & 0x7FFFu) != 0u;
     }
     if (__ballot(mism) != 0ull && lane == 0) {
       if (BATCH) atomicOr(a.doc_flags + pv.doc, 2u);
@@ -1821,7 +1939,7 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
     for (int i = 0; i < PK_IPT; ++i) {
       ob[i] = b;
       op[i] = q;
-      pos[lane * PK_IPT + i] = make_uint4((uint32_t)b, (uint32_t)d, (uint32_t)(inf[i].sst + cl[i]), 0u);
+      pos[lane * PK_IPT + i] = make_uint4((uint32_t)b, (uint32_t)d, (uint32_t)(inf[i].sst + skip[i]), 0u);
       b += nbw[i];
       q += inf[i].np;
       d += inf[i].dd;
@@ -2155,7 +2273,7 @@ struct alpacc_ctx {
   EntInfo *d_einfo = nullptr;
   uint32_t *d_stackw = nullptr;
   uint2 *d_prodw = nullptr;
-  uint16_t *d_ecan = nullptr;  // per entry: trailing left brackets | leading right brackets << 8
+  uint32_t *d_ecan = nullptr;  // per entry: trailing lefts | leading rights << 8 | brackets between << 16
   uint32_t max_nb = 0, max_np = 0;
   size_t nstackw = 0, nprodw = 0;
   // scratch (grow-only)
@@ -2936,7 +3054,7 @@ bool build_parser_tables(alpacc_ctx *ctx, const alpacc_tables *t, std::string &e
   std::vector<EntInfo> info(E);
   std::vector<uint32_t> stackw;
   std::vector<uint2> prodw;
-  std::vector<uint16_t> ecan(E);
+  std::vector<uint32_t> ecan(E);
   for (uint32_t e = 0; e < E; ++e) {
     const uint32_t *e4 = t->pair_entry + 4ull * e;
     if (e4[1] < e4[0] || e4[3] < e4[2] || e4[1] > t->stacks_len || e4[3] > t->productions_len) {
@@ -2948,7 +3066,8 @@ bool build_parser_tables(alpacc_ctx *ctx, const alpacc_tables *t, std::string &e
     uint32_t lr = 0, tl = 0;
     while (lr < 255 && e4[0] + lr < e4[1] && !(t->stacks[e4[0] + lr] & 0x8000u)) ++lr;
     while (tl < 255 && e4[1] - tl > e4[0] && (t->stacks[e4[1] - 1 - tl] & 0x8000u)) ++tl;
-    ecan[e] = (uint16_t)(tl | (lr << 8));
+    // bit 16: brackets between the counted leading rights and trailing lefts (they never cancel)
+    ecan[e] = tl | (lr << 8) | ((e4[1] - e4[0] > lr + tl) ? 0x10000u : 0u);
     EntInfo &f = info[e];
     f.nb = (uint16_t)(e4[1] - e4[0]);
     f.np = (uint16_t)(e4[3] - e4[2]);

@@ -3,10 +3,12 @@ else: every key has a pair entry and the depths are valid, but some right bracke
 left bracket at its previous smaller-or-equal depth.
 
 The device cancels a position's trailing left brackets against the next position's leading right
-brackets when it expands the bracket stream (pairs of positions inside one 256-position group of a
-1024-position tile), checks those pairs there, and leaves the rest to the bracket search.
-``classify`` names where a mismatch sits, so that the tests can show both places are exercised:
-"adjacent" (a cancelled pair mismatches) or "remaining" (only the search sees it). It classifies
+brackets when it expands the bracket stream, then the remaining trailing lefts against the
+remaining leading rights of the next position that still has brackets (positions inside one
+256-position group of a 1024-position tile), checks those pairs there, and leaves the rest to the
+bracket search. ``classify`` names where a mismatch sits, so that the tests can show every place
+is exercised: "adjacent" (a first-level pair mismatches), "second" (a second-level pair) or
+"remaining" (only the search sees it). It classifies
 with the literal restatement of parser.fut; the verdicts themselves are compared with the C
 restatement.
 
@@ -62,7 +64,7 @@ def boundary_doc(k: int) -> bytes:
 
 
 def classify(bundle, kinds) -> str:
-    """"key", "depth", "adjacent", "remaining" or "accept" for a token kind stream."""
+    """"key", "depth", "adjacent", "second", "remaining" or "accept" for a token kind stream."""
     ps = ParserFut(bundle.parser)
     spans = ps.lookup(ps.keys(np.asarray(kinds, dtype=np.int64)))
     if (spans == -1).any():
@@ -77,19 +79,39 @@ def classify(bundle, kinds) -> str:
             return "depth"
     if ps.brackets_matches(flat):
         return "accept"
-    for p in range(len(seqs) - 1):
+    n = len(seqs)
+    lr = [0] * n
+    tl = [0] * n
+    for p, s in enumerate(seqs):
+        while lr[p] < len(s) and not (s[lr[p]] & top):
+            lr[p] += 1
+        while tl[p] < len(s) and s[len(s) - 1 - tl[p]] & top:
+            tl[p] += 1
+    # first level: a position's trailing lefts against the next position's leading rights
+    cr = [0] * n
+    for p in range(n - 1):
         if p // GROUP != (p + 1) // GROUP:
             continue
+        cr[p] = min(tl[p], lr[p + 1])
         a, b = seqs[p], seqs[p + 1]
-        tl = 0
-        while tl < len(a) and a[len(a) - 1 - tl] & top:
-            tl += 1
-        lr = 0
-        while lr < len(b) and not (b[lr] & top):
-            lr += 1
-        for j in range(min(tl, lr)):
+        for j in range(cr[p]):
             if (a[len(a) - 1 - j] & ~top) != (b[j] & ~top):
                 return "adjacent"
+    # second level: the remaining trailing lefts against the remaining leading rights of the next
+    # position of the group that still has brackets
+    cl = [0] + cr[:-1]
+    left = [len(seqs[p]) - cl[p] - cr[p] for p in range(n)]
+    for p in range(n):
+        q = p + 1
+        while q < n and q // GROUP == p // GROUP and left[q] == 0:
+            q += 1
+        if q >= n or q // GROUP != p // GROUP:
+            continue
+        c2 = min(tl[p] - cr[p], lr[q] - cl[q])
+        a, b = seqs[p], seqs[q]
+        for j in range(c2):
+            if (a[len(a) - 1 - cr[p] - j] & ~top) != (b[cl[q] + j] & ~top):
+                return "second"
     return "remaining"
 
 
@@ -139,7 +161,7 @@ def token_mutations(bundle, doc: bytes, seed: int, trials: int, far: bool = Fals
         else:
             k2[int(rng.integers(n))] = int(rng.integers(nk))
         lab = classify(bundle, k2)
-        if lab in ("adjacent", "remaining"):
+        if lab in ("adjacent", "second", "remaining"):
             out.append((k2.astype(np.uint32), starts.astype(np.uint32), ends.astype(np.uint32), lab))
     return out
 
@@ -163,6 +185,6 @@ def byte_swaps(bundle, doc: bytes, seed: int, trials: int):
         if r is None or not np.array_equal(np.asarray(r[0]).astype(np.int64), k2):
             continue
         lab = classify(bundle, k2)
-        if lab in ("adjacent", "remaining"):
+        if lab in ("adjacent", "second", "remaining"):
             out.append((d2, lab))
     return out

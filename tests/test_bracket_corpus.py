@@ -1,6 +1,7 @@
 """The bracket-mismatch corpus (bracket_corpus.py) on the CPU: its labels agree with the C
-restatement of parser.fut, and it holds mismatches of both kinds, in a cancelled pair and among
-the remaining brackets, including one straddling two 256-position groups."""
+restatement of parser.fut, and it holds mismatches of every kind: in a first-level or a
+second-level cancelled pair and among the remaining brackets, including one straddling two
+256-position groups."""
 import os
 from collections import Counter
 
@@ -12,8 +13,8 @@ from conftest import GRAMMARS
 
 # grammar, longest stream, labels that must occur (arithmetic's keys and depths catch every short
 # fault: 37050 key misses and 200 depth faults among its 37448 streams up to length 5)
-CASES = (("sexp", 7, ("accept", "adjacent", "remaining")),
-         ("two_brackets", 5, ("accept", "adjacent", "remaining")),
+CASES = (("sexp", 7, ("accept", "adjacent", "second", "remaining")),
+         ("two_brackets", 5, ("accept", "adjacent", "second", "remaining")),
          ("arithmetic", 4, ("accept", "key", "depth")))
 
 
