@@ -1282,11 +1282,10 @@ __device__ __forceinline__ int32_t pk_cancelled(const int32_t (&cr)[PK_IPT], int
 // have none left) follow each other directly as well: min(lefts, rights) more pairs. Inside the
 // wavefront: the next such position in another lane comes through a ballot and a permute.
 // Per position: n1 brackets left after the adjacent cancellation, r1 leading rights and l1
-// trailing lefts among them, f1 the stackw index of its first remaining bracket (expand only).
+// trailing lefts among them, f1 the stackw index of its first remaining bracket.
 // Out: cr2[i] pairs position i cancels with its next position that has brackets left, nf[i] that
-// position's f1; with WANT_CL also cl2[i], the pairs position i cancels with its previous one.
-// All lanes of the wavefront take part.
-template <bool WANT_CL>
+// position's f1, cl2[i] the pairs position i cancels with its previous one. All lanes of the
+// wavefront take part. (parse_keys2 counts the same pairs with a leaner copy of the first half.)
 __device__ __forceinline__ void pk_cancel2(const int32_t (&n1)[PK_IPT],
                                            const int32_t (&r1)[PK_IPT],
                                            const int32_t (&l1)[PK_IPT],
@@ -1325,24 +1324,22 @@ __device__ __forceinline__ void pk_cancel2(const int32_t (&n1)[PK_IPT],
     cr2[i] = min(l1[i], nr);
     nf[i] = nfi;
   }
-  if (WANT_CL) {
-    // the lane's last position with brackets left: the pairs it cancels forward
-    int32_t lc = 0;
+  // the lane's last position with brackets left: the pairs it cancels forward
+  int32_t lc = 0;
 #pragma unroll
-    for (int i = 0; i < PK_IPT; ++i)
-      if (n1[i] > 0) lc = cr2[i];
-    const uint64_t below = mask & ((1ull << lane) - 1ull);
-    const int pl = below ? 63 - (int)__builtin_clzll(below) : lane;
-    const int32_t xc_all = __shfl(lc, pl, WAVE);
-    const int32_t xc = below ? xc_all : 0;
+  for (int i = 0; i < PK_IPT; ++i)
+    if (n1[i] > 0) lc = cr2[i];
+  const uint64_t below = mask & ((1ull << lane) - 1ull);
+  const int pl = below ? 63 - (int)__builtin_clzll(below) : lane;
+  const int32_t xc_all = __shfl(lc, pl, WAVE);
+  const int32_t xc = below ? xc_all : 0;
 #pragma unroll
-    for (int k = 0; k < PK_IPT; ++k) {
-      int32_t pc = xc;
+  for (int k = 0; k < PK_IPT; ++k) {
+    int32_t pc = xc;
 #pragma unroll
-      for (int i = 0; i < k; ++i)
-        if (n1[i] > 0) pc = cr2[i];
-      cl2[k] = n1[k] > 0 ? pc : 0;
-    }
+    for (int i = 0; i < k; ++i)
+      if (n1[i] > 0) pc = cr2[i];
+    cl2[k] = n1[k] > 0 ? pc : 0;
   }
 }
 
@@ -1551,7 +1548,7 @@ parse_keys2(PArgs a) {
       int32_t cr[PK_IPT], cl0;
       pk_cancel(tlr, cr, cl0);
       acc.nb -= pk_cancelled(cr, cl0);
-      // second level (pk_cancel2; only the pair count is needed here): per position the leading
+      // second level (as pk_cancel2; only the pair count is needed here): per position the leading
       // rights and trailing lefts left after the first level, and whether any bracket is left,
       // packed into one word so that the tile's registers stay within 8 waves per SIMD
       uint32_t lv[PK_IPT];
@@ -1895,7 +1892,7 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
     l1[i] = (int32_t)(tlr[i] & 0xFFu) - cr[i];
     f1[i] = (uint32_t)inf[i].sst + (uint32_t)cl[i];
   }
-  pk_cancel2<true>(nbw, r1, l1, f1, lane, cr2, nf, cl2);
+  pk_cancel2(nbw, r1, l1, f1, lane, cr2, nf, cl2);
 #pragma unroll
   for (int i = 0; i < PK_IPT; ++i) {
     // the lane's own brackets (its positions' offsets come from a scan over the lanes; the tile
