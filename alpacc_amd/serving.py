@@ -5,7 +5,9 @@ src/Alpacc/Generator/Futhark/Generator.hs:35-44). This loop keeps one grammar's 
 and serves a stream of documents:
 
 * every worker owns a context (tables in HBM, grow-only scratch arenas), a non-blocking stream and
-  pinned staging buffers;
+  pinned staging buffers for the documents going in and the trees coming back (a device-to-host
+  copy into pinned memory runs at DMA speed; the fresh per-request arrays are then filled by a
+  pool of host threads);
 * a worker that becomes free takes the requests waiting at that moment, in arrival order, as long
   as their estimated HBM footprint fits the shared budget (admission by footprint) and the batch
   limits hold, and runs them as one ``alpacc_parse_batch`` call. A batch holds requests of one
@@ -27,6 +29,7 @@ from __future__ import annotations
 import threading
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -41,6 +44,20 @@ FOOTPRINT_FIXED = 1 << 20
 
 
 _CREATE_LOCK = threading.Lock()  # alpacc_create reports errors through one global string
+_COPY_POOL = ThreadPoolExecutor(max_workers=8)  # host copies of large trees (numpy drops the GIL)
+_COPY_SPLIT = 16 << 20                          # bytes per copy task
+
+
+def _copy_into(dst: np.ndarray, src: np.ndarray) -> None:
+    """dst[:] = src (uint8), large ones in parallel slices: one thread writing fresh pages moves
+    about 9 GB/s, and a 64 MiB JSON request's tree is about 250 MB."""
+    n = src.shape[0]
+    if n <= _COPY_SPLIT:
+        np.copyto(dst, src)
+        return
+    cuts = list(range(0, n, _COPY_SPLIT)) + [n]
+    list(_COPY_POOL.map(lambda ab: np.copyto(dst[ab[0]:ab[1]], src[ab[0]:ab[1]]),
+                        zip(cuts[:-1], cuts[1:])))
 
 
 def footprint(nbytes: int) -> int:
@@ -152,7 +169,7 @@ class Server:
                 ctx = Context(self.bundle, self.device)
             stream = lib.alpacc_stream_create()
             ctx.stream = stream
-            state = {"in": None, "pin_in": None, "out": None, "ratio": 1.0}
+            state = {"in": None, "pin_in": None, "out": None, "pin_out": None, "ratio": 1.0}
             if self._warm is not None:
                 self._run(ctx, lib, stream, state, [Request(-1, self._warm)], -1)
         except Exception as exc:  # reported to the constructor
@@ -190,8 +207,9 @@ class Server:
         for k in ("in", "out"):
             for b in (state[k] or []):
                 b.free()
-        if state["pin_in"] is not None:
-            state["pin_in"].free()
+        for k in ("pin_in", "pin_out"):
+            if state[k] is not None:
+                state[k].free()
         ctx.close()
         lib.alpacc_stream_destroy(stream)
 
@@ -222,7 +240,7 @@ class Server:
         total = int(lengths.sum())
         pin = self._grow_pin(state, "pin_in", total, lib)
         for r, o in zip(batch, offsets.tolist()):
-            pin.array[o:o + r.data.shape[0]] = r.data
+            _copy_into(pin.array[o:o + r.data.shape[0]], r.data)
         t_staged = time.perf_counter()
         (d_in,) = self._grow_dev(state, "in", [total], lib)
         if total and lib.alpacc_copy_to_device(d_in.ptr, pin.ptr, total, stream) != 0:
@@ -239,14 +257,19 @@ class Server:
         t_parsed = time.perf_counter()
         if total:
             state["ratio"] = max(state["ratio"] * 0.9, nodes / total)
-        # the batch's trees come back into fresh arrays; every request gets views of its range
-        par = np.empty(nodes, dtype=np.uint32)
-        nd = np.empty(nodes, dtype=np.uint32)
-        sp = np.empty((nodes, 2), dtype=np.uint32)
+        # the batch's trees come back through pinned memory into fresh arrays; every request gets
+        # views of its range
+        pin = self._grow_pin(state, "pin_out", 16 * nodes, lib)
         if nodes:
-            for dst, b in ((par, outs[0]), (nd, outs[1]), (sp, outs[2])):
-                if lib.alpacc_copy_to_host(dst.ctypes.data, b.ptr, dst.nbytes, stream) != 0:
+            for off, b, nb in ((0, outs[0], 4 * nodes), (4 * nodes, outs[1], 4 * nodes),
+                               (8 * nodes, outs[2], 8 * nodes)):
+                if lib.alpacc_copy_to_host(pin.ptr + off, b.ptr, nb, stream) != 0:
                     raise RuntimeError("device to host copy failed")
+        mem = np.empty(16 * nodes, dtype=np.uint8)
+        _copy_into(mem, pin.array[:16 * nodes])
+        par = mem[:4 * nodes].view(np.uint32)
+        nd = mem[4 * nodes:8 * nodes].view(np.uint32)
+        sp = mem[8 * nodes:].view(np.uint32).reshape(-1, 2)
         now = time.perf_counter()
         for r, x in zip(batch, res):
             r.status = int(x["status"])
