@@ -73,3 +73,16 @@ def test_workload_shapes():
     assert abs(np.log(np.median(sz)) - 0.5 * (np.log(256) + np.log(64 << 20))) < 0.2
     arr = serving.poisson_arrivals(10_000, rate=100.0, seed=1)
     assert np.all(np.diff(arr) > 0) and abs(arr[-1] / 10_000 - 0.01) < 0.001
+
+
+def test_parallel_copy():
+    """The serving loop's host copies: small ones directly, large ones in parallel slices; every
+    byte lands, at any offset and length."""
+    from alpacc_amd import serving
+    rng = np.random.Generator(np.random.PCG64(5))
+    src = rng.integers(0, 256, size=(serving._COPY_SPLIT * 3) + 12345, dtype=np.uint8)
+    for n in (0, 1, serving._COPY_SPLIT, serving._COPY_SPLIT + 1, src.shape[0]):
+        dst = np.zeros(src.shape[0] + 7, dtype=np.uint8)
+        serving._copy_into(dst[3:3 + n], src[:n])
+        assert np.array_equal(dst[3:3 + n], src[:n])
+        assert not dst[:3].any() and not dst[3 + n:].any()
