@@ -29,12 +29,11 @@ from __future__ import annotations
 import threading
 import time
 from collections import deque
-from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
 
-from .runtime import ACCEPT, Context, DeviceBuffer, HostBuffer, load_library
+from .runtime import ACCEPT, Context, DeviceBuffer, HostBuffer, copy_into, load_library
 
 # Device bytes per input byte while a request is in flight: the input, the token slots
 # (3 x u32 per byte), key positions and pair entries, brackets / productions with their scan
@@ -44,20 +43,6 @@ FOOTPRINT_FIXED = 1 << 20
 
 
 _CREATE_LOCK = threading.Lock()  # alpacc_create reports errors through one global string
-_COPY_POOL = ThreadPoolExecutor(max_workers=8)  # host copies of large trees (numpy drops the GIL)
-_COPY_SPLIT = 16 << 20                          # bytes per copy task
-
-
-def _copy_into(dst: np.ndarray, src: np.ndarray) -> None:
-    """dst[:] = src (uint8), large ones in parallel slices: one thread writing fresh pages moves
-    about 9 GB/s, and a 64 MiB JSON request's tree is about 250 MB."""
-    n = src.shape[0]
-    if n <= _COPY_SPLIT:
-        np.copyto(dst, src)
-        return
-    cuts = list(range(0, n, _COPY_SPLIT)) + [n]
-    list(_COPY_POOL.map(lambda ab: np.copyto(dst[ab[0]:ab[1]], src[ab[0]:ab[1]]),
-                        zip(cuts[:-1], cuts[1:])))
 
 
 def footprint(nbytes: int) -> int:
@@ -240,7 +225,7 @@ class Server:
         total = int(lengths.sum())
         pin = self._grow_pin(state, "pin_in", total, lib)
         for r, o in zip(batch, offsets.tolist()):
-            _copy_into(pin.array[o:o + r.data.shape[0]], r.data)
+            copy_into(pin.array[o:o + r.data.shape[0]], r.data)
         t_staged = time.perf_counter()
         (d_in,) = self._grow_dev(state, "in", [total], lib)
         if total and lib.alpacc_copy_to_device(d_in.ptr, pin.ptr, total, stream) != 0:
@@ -266,7 +251,7 @@ class Server:
                 if lib.alpacc_copy_to_host(pin.ptr + off, b.ptr, nb, stream) != 0:
                     raise RuntimeError("device to host copy failed")
         mem = np.empty(16 * nodes, dtype=np.uint8)
-        _copy_into(mem, pin.array[:16 * nodes])
+        copy_into(mem, pin.array[:16 * nodes])
         par = mem[:4 * nodes].view(np.uint32)
         nd = mem[4 * nodes:8 * nodes].view(np.uint32)
         sp = mem[8 * nodes:].view(np.uint32).reshape(-1, 2)

@@ -76,13 +76,13 @@ def test_workload_shapes():
 
 
 def test_parallel_copy():
-    """The serving loop's host copies: small ones directly, large ones in parallel slices; every
-    byte lands, at any offset and length."""
-    from alpacc_amd import serving
+    """The host copies of the serving loop and of Context's large transfers: small ones directly,
+    large ones in parallel slices; every byte lands, at any offset and length."""
+    from alpacc_amd import runtime
     rng = np.random.Generator(np.random.PCG64(5))
-    src = rng.integers(0, 256, size=(serving._COPY_SPLIT * 3) + 12345, dtype=np.uint8)
-    for n in (0, 1, serving._COPY_SPLIT, serving._COPY_SPLIT + 1, src.shape[0]):
+    src = rng.integers(0, 256, size=(runtime.COPY_SPLIT * 3) + 12345, dtype=np.uint8)
+    for n in (0, 1, runtime.COPY_SPLIT, runtime.COPY_SPLIT + 1, src.shape[0]):
         dst = np.zeros(src.shape[0] + 7, dtype=np.uint8)
-        serving._copy_into(dst[3:3 + n], src[:n])
+        runtime.copy_into(dst[3:3 + n], src[:n])
         assert np.array_equal(dst[3:3 + n], src[:n])
         assert not dst[:3].any() and not dst[3 + n:].any()
