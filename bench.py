@@ -47,7 +47,9 @@ def parse_args():
                     help="only run device-to-device copies of this size (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--serve-requests", type=int, default=300)
-    ap.add_argument("--serve-rate", type=float, default=400.0, help="offered load, requests/s")
+    ap.add_argument("--serve-rate", type=float, default=1200.0,
+                    help="offered load, requests/s (1200/s = 5.7 GB/s: past what one request per "
+                         "call sustains)")
     ap.add_argument("--serve-workers", type=int, default=4)
     return ap.parse_args()
 
