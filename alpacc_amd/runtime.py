@@ -345,21 +345,30 @@ class Context:
             buf.upload(arr, self.stream)
         return buf, n
 
-    def download_tree(self, pb: DeviceBuffer, nb: DeviceBuffer, spb: DeviceBuffer, nn: int):
-        """(parent, node, span[N, 2]) u32 arrays of N = nn nodes from the device buffers; large
-        trees come through pinned memory."""
-        if 16 * nn < PINNED_MIN:
-            return (pb.download(np.uint32, nn, self.stream), nb.download(np.uint32, nn, self.stream),
-                    spb.download(np.uint32, 2 * nn, self.stream).reshape(-1, 2))
-        pin = self.pinned("tree", 16 * nn)
-        for off, b, size in ((0, pb, 4 * nn), (4 * nn, nb, 4 * nn), (8 * nn, spb, 8 * nn)):
-            if self.lib.alpacc_copy_to_host(ctypes.c_void_p(pin.ptr + off), ctypes.c_void_p(b.ptr),
-                                            size, self.stream) != 0:
+    def download_arrays(self, parts) -> list:
+        """[(DeviceBuffer, dtype, count), ...] -> host arrays, one fresh allocation for all of them;
+        from PINNED_MIN bytes up through pinned staging (DMA) and parallel host copies."""
+        sizes = [np.dtype(dt).itemsize * int(c) for _, dt, c in parts]
+        offs = [0]
+        for size in sizes:
+            offs.append(offs[-1] + ((size + 15) & ~15))  # every part 16-byte aligned
+        total = offs[-1]
+        if sum(sizes) < PINNED_MIN:
+            return [b.download(dt, int(c), self.stream) for b, dt, c in parts]
+        pin = self.pinned("out", total)
+        for (b, _, _), size, off in zip(parts, sizes, offs):
+            if size and self.lib.alpacc_copy_to_host(ctypes.c_void_p(pin.ptr + off),
+                                                     ctypes.c_void_p(b.ptr), size, self.stream) != 0:
                 raise RuntimeError("device to host copy failed")
-        mem = np.empty(16 * nn, dtype=np.uint8)
-        copy_into(mem, pin.array[:16 * nn])
-        return (mem[:4 * nn].view(np.uint32), mem[4 * nn:8 * nn].view(np.uint32),
-                mem[8 * nn:].view(np.uint32).reshape(-1, 2))
+        mem = np.empty(total, dtype=np.uint8)
+        copy_into(mem, pin.array[:total])
+        return [mem[off:off + size].view(dt) for (_, dt, _), size, off in zip(parts, sizes, offs)]
+
+    def download_tree(self, pb: DeviceBuffer, nb: DeviceBuffer, spb: DeviceBuffer, nn: int):
+        """(parent, node, span[N, 2]) u32 arrays of N = nn nodes from the device buffers."""
+        par, node, span = self.download_arrays(((pb, np.uint32, nn), (nb, np.uint32, nn),
+                                                (spb, np.uint32, 2 * nn)))
+        return par, node, span.reshape(-1, 2)
 
     # -- profiling ---------------------------------------------------------------------------
     def profile(self, on: bool = True):
@@ -437,8 +446,8 @@ class Context:
         rc, nt = self.lex_device(d_in.ptr, n, kb.ptr, sb.ptr, eb.ptr, cap, self.stream)
         if rc != ACCEPT:
             return None
-        return (kb.download(np.uint32, nt, self.stream), sb.download(np.uint32, nt, self.stream),
-                eb.download(np.uint32, nt, self.stream))
+        return tuple(self.download_arrays(((kb, np.uint32, nt), (sb, np.uint32, nt),
+                                           (eb, np.uint32, nt))))
 
     def _lex_int_chunked(self, data, chunk: int):
         d_in, n = self.upload_doc(data)
@@ -450,8 +459,8 @@ class Context:
         rc, nt = self.lex_chunked_device(d_in.ptr, n, chunk, kb.ptr, sb.ptr, eb.ptr, cap, self.stream)
         if rc != ACCEPT:
             return None
-        return (kb.download(np.uint32, nt, self.stream), sb.download(np.uint64, nt, self.stream),
-                eb.download(np.uint64, nt, self.stream))
+        return tuple(self.download_arrays(((kb, np.uint32, nt), (sb, np.uint64, nt),
+                                           (eb, np.uint64, nt))))
 
     def lex(self, data, chunk_size: int | None = None):
         r = self.lex_int(data, chunk_size)
