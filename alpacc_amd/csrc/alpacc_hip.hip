@@ -933,6 +933,9 @@ __global__ void lex_chunk_expand(const uint8_t *in, uint32_t chunk, uint64_t nv,
   }
 }
 
+// one word set from the device (no memset node, so that a captured graph replays it)
+__global__ void set_u32(uint32_t *p, uint32_t v) { *p = v; }
+
 __global__ void clear_u32(uint32_t *p, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -1689,13 +1692,20 @@ __global__ void __launch_bounds__(PSB) pscan_reduce(PArgs a, SegTup *btot) {
 __global__ void __launch_bounds__(PSB) pscan_top(PArgs a, SegTup *btot, uint64_t nblocks_max) {
   __shared__ SegTup s_w[PSB / WAVE];
   __shared__ SegTup s_carry;
+  __shared__ unsigned long long s_elems[2];  // brackets, productions over all blocks (64-bit)
   const uint64_t ntiles = a.ntiles_dev ? *a.ntiles_dev : pk_ntiles<false>(a);
   const uint64_t nb = std::min<uint64_t>(nblocks_max, (ntiles + PSB - 1) / PSB);
-  if (threadIdx.x == 0) s_carry = SegTup{ptup_id(), 0};
+  if (threadIdx.x == 0) {
+    s_carry = SegTup{ptup_id(), 0};
+    s_elems[0] = s_elems[1] = 0ull;
+  }
   __syncthreads();
+  unsigned long long mine_b = 0ull, mine_p = 0ull;
   for (uint64_t off = 0; off < nb; off += PSB) {
     const uint64_t i = off + threadIdx.x;
     const SegTup v = i < nb ? btot[i] : SegTup{ptup_id(), 0};
+    mine_b += (uint32_t)v.t.nb;
+    mine_p += (uint32_t)v.t.np;
     SegTup total;
     const SegTup ex = seg_block_excl(v, s_w, total);
     const SegTup carry = s_carry;
@@ -1704,7 +1714,16 @@ __global__ void __launch_bounds__(PSB) pscan_top(PArgs a, SegTup *btot, uint64_t
     if (threadIdx.x == 0) s_carry = seg_op(carry, total);
     __syncthreads();
   }
-  if (threadIdx.x == 0) *a.total = s_carry.t;
+  // element offsets are 32-bit: flag a grand total of brackets or productions of 2^31 or more
+  // (the int32 sums above would have wrapped). Every block total is below 2^31 (at most
+  // PSB * PK_TILE positions of fewer than 2048 elements each, checked on the host).
+  atomicAdd(&s_elems[0], mine_b);
+  atomicAdd(&s_elems[1], mine_p);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *a.total = s_carry.t;
+    if (s_elems[0] >= (1ull << 31) || s_elems[1] >= (1ull << 31)) a.flags[1] = 1u;
+  }
 }
 
 template <bool BATCH>
@@ -2252,6 +2271,8 @@ struct alpacc_ctx {
   int cus = 0;
   std::string err;
   int launches = 0;
+  // testing hook: raise the lexer's look-back time-out flag after it is cleared
+  bool force_timeout = false;
   // lexer tables
   bool has_lexer = false;
   uint32_t e = 0, C = 0, m1 = 0, init = 0, dead = 0;
@@ -2501,6 +2522,7 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
     return ALPACC_ERR_HIP;
   }
   clear_u32<<<1, 64, 0, st>>>(small, 16);
+  if (ctx->force_timeout) set_u32<<<1, 1, 0, st>>>(small + 6, 1u);
   clear_u32<<<grid_for(2 * ctiles, 256), 256, 0, st>>>((uint32_t *)st_count, 2 * ctiles);
   LexArgs a;
   a.in = d_in;
@@ -2612,6 +2634,7 @@ int run_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
   *d_docs_out = d_docs;
   clear_u32<<<grid_for(2 * (uint64_t)ndocs, 256), 256, 0, st>>>(doc_res, 2 * (uint64_t)ndocs);
   clear_u32<<<1, 64, 0, st>>>(small, 16);
+  if (ctx->force_timeout) set_u32<<<1, 1, 0, st>>>(small + 6, 1u);
   ctx->launches += 2;
   if (tiles == 0) {
     CK(hipGetLastError());
@@ -2943,6 +2966,14 @@ int run_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offset
     ctx->err = "batch too large: token slots and key positions must stay below 2^32";
     return ALPACC_ERR_ARG;
   }
+  // element offsets are 32-bit. pscan_top flags a total of 2^31 or more on the device, which is
+  // exact while every block of PSB tiles stays below 2^31 elements; with pair entries of 2048
+  // elements or more that is not certain, and the host bound (T + 2 <= length + 2) decides.
+  const uint64_t elem_max = (uint64_t)std::max(ctx->max_nb, ctx->max_np);
+  if (elem_max >= 2048 && (tslots + 2ull * ndocs) * elem_max >= (1ull << 31)) {
+    ctx->err = "batch too large for 32-bit element counts";
+    return ALPACC_ERR_ARG;
+  }
   uint32_t *kinds = (uint32_t *)ctx->buf(B_TOK_KIND, tslots * 4);
   uint32_t *starts = (uint32_t *)ctx->buf(B_TOK_START, tslots * 4);
   uint32_t *ends = (uint32_t *)ctx->buf(B_TOK_END, tslots * 4);
@@ -2989,13 +3020,19 @@ int run_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offset
   pb_docs<<<(ndocs + 255) / 256, 256, 0, st>>>(a);
   ctx->launches++;
   CK(hipGetLastError());
-  CK(hipMemcpyAsync(ctx->h_small, small + 16, 12 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  // words 4..7: the lexer's results (time-out flag at 6); 8..11: parser flags (overflow at 9);
+  // 16..21: grand total; 24..25: position tiles
+  CK(hipMemcpyAsync(ctx->h_small, small, 28 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   CK(hipStreamSynchronize(st));
+  if (ctx->h_small[6]) {
+    ctx->err = "lexer look-back did not complete";
+    return ALPACC_ERR_TIMEOUT;
+  }
   PTup tot;
-  memcpy(&tot, ctx->h_small, sizeof(PTup));
+  memcpy(&tot, ctx->h_small + 16, sizeof(PTup));
   uint64_t ntiles;
-  memcpy(&ntiles, ctx->h_small + 8, sizeof(uint64_t));
-  if (tot.nb < 0 || tot.np < 0) {
+  memcpy(&ntiles, ctx->h_small + 24, sizeof(uint64_t));
+  if (ctx->h_small[9] || tot.nb < 0 || tot.np < 0) {
     ctx->err = "batch too large: more than 2^31 brackets or productions";
     return ALPACC_ERR_ARG;
   }
@@ -3256,6 +3293,12 @@ const char *alpacc_last_error(alpacc_ctx *ctx) { return ctx ? ctx->err.c_str() :
 
 int alpacc_last_launch_count(alpacc_ctx *ctx) { return ctx ? ctx->launches : 0; }
 
+int alpacc_test_force_timeout(alpacc_ctx *ctx, int on) {
+  if (!ctx) return ALPACC_ERR_ARG;
+  ctx->force_timeout = on != 0;
+  return 0;
+}
+
 int alpacc_last_counts(alpacc_ctx *ctx, uint64_t *counts) {
   if (!ctx || !counts) return ALPACC_ERR_ARG;
   for (int i = 0; i < 4; ++i) counts[i] = ctx->last_counts[i];
@@ -3449,11 +3492,18 @@ int alpacc_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offse
   if (rc == 0) {
     lb_finish<<<(num_docs + 255) / 256, 256, 0, st>>>(ldocs, doc_res, num_docs, d_results);
     ctx->launches++;
+    uint32_t *small = (uint32_t *)ctx->buf(B_SMALL, 64 * sizeof(uint32_t));
     if (hipMemcpyAsync(results, d_results, num_docs * sizeof(alpacc_doc_result),
                        hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(ctx->h_small, small + 4, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                       st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess) {
       ctx->err = "HIP error in the batched lexer";
       rc = ALPACC_ERR_HIP;
+    } else if (ctx->h_small[2]) {
+      // a bounded look-back wait gave up: the offsets behind it are wrong
+      ctx->err = "lexer look-back did not complete";
+      rc = ALPACC_ERR_TIMEOUT;
     } else {
       rc = ALPACC_ACCEPT;
     }
@@ -3596,13 +3646,20 @@ int test_bytes(alpacc_ctx *ctx, const std::vector<std::pair<uint64_t, uint64_t>>
                                                                        (uint32_t)small.size(),
                                                                        d_results);
       std::vector<uint32_t> k(cap), sa(cap), ea(cap);
+      uint32_t *dsmall = (uint32_t *)ctx->buf(B_SMALL, 64 * sizeof(uint32_t));
       if (hipMemcpyAsync(res.data(), d_results, small.size() * sizeof(alpacc_doc_result),
                          hipMemcpyDeviceToHost, st) != hipSuccess ||
           hipMemcpyAsync(k.data(), p0, cap * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
           hipMemcpyAsync(sa.data(), p1, cap * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
           hipMemcpyAsync(ea.data(), p2, cap * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipMemcpyAsync(ctx->h_small, dsmall + 4, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                         st) != hipSuccess ||
           hipStreamSynchronize(st) != hipSuccess)
         return ALPACC_ERR_HIP;
+      if (ctx->h_small[2]) {
+        ctx->err = "lexer look-back did not complete";
+        return ALPACC_ERR_TIMEOUT;
+      }
       for (size_t j = 0; j < small.size(); ++j) {
         std::vector<uint8_t> &o = rec[small[j]];
         if (res[j].status != ALPACC_ACCEPT) continue;

@@ -100,6 +100,9 @@ class Server:
             t.start()
         self._ready.wait()
         if self._errors:
+            # the workers that did start wait for work: stop them, so that each frees its
+            # context, stream and pinned buffers, before the error goes to the caller
+            self.close()
             raise self._errors[0]
 
     # -- client side ----------------------------------------------------------------------------
@@ -148,17 +151,19 @@ class Server:
         return batch
 
     def _worker(self, w: int):
+        ctx = stream = None
+        state = {"in": None, "pin_in": None, "out": None, "pin_out": None, "ratio": 1.0}
         try:
             lib = load_library()
             with _CREATE_LOCK:
                 ctx = Context(self.bundle, self.device)
             stream = lib.alpacc_stream_create()
             ctx.stream = stream
-            state = {"in": None, "pin_in": None, "out": None, "pin_out": None, "ratio": 1.0}
             if self._warm is not None:
                 self._run(ctx, lib, stream, state, [Request(-1, self._warm)], -1)
         except Exception as exc:  # reported to the constructor
             self._errors.append(exc)
+            self._release(ctx, stream, state)
             self._ready.wait()
             return
         self._ready.wait()
@@ -189,14 +194,23 @@ class Server:
                 with self._cv:
                     self._inflight -= sum(footprint(r.data.shape[0]) for r in batch)
                     self._cv.notify_all()
+        self._release(ctx, stream, state)
+
+    @staticmethod
+    def _release(ctx, stream, state):
+        """Free a worker's buffers, context and stream (whatever of them exists)."""
         for k in ("in", "out"):
             for b in (state[k] or []):
                 b.free()
+            state[k] = None
         for k in ("pin_in", "pin_out"):
             if state[k] is not None:
                 state[k].free()
-        ctx.close()
-        lib.alpacc_stream_destroy(stream)
+            state[k] = None
+        if ctx is not None:
+            ctx.close()
+        if stream is not None:
+            load_library().alpacc_stream_destroy(stream)
 
     @staticmethod
     def _grow_dev(state, key, sizes, lib):

@@ -168,6 +168,12 @@ void alpacc_free(void *ptr);
 /* Number of kernels the last call on `ctx` launched. */
 int alpacc_last_launch_count(alpacc_ctx *ctx);
 
+/* Testing hook (no counterpart in the reference): with on != 0, every lexer pass of the following
+ * calls on `ctx` raises its look-back time-out flag on the device right after clearing it, as a
+ * bounded look-back wait that gave up would. Every kernel still runs to its normal end; the call
+ * must then return ALPACC_ERR_TIMEOUT instead of results built on a wrong prefix. */
+int alpacc_test_force_timeout(alpacc_ctx *ctx, int on);
+
 /* Element counts of the last parser call on `ctx`: counts[0] tokens, [1] key positions (T + 2;
  * tile-padded in batched calls), [2] brackets written for the bracket search (those left after
  * neighbouring positions' matching brackets cancel), [3] productions (tree nodes). */
