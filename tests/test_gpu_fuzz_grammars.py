@@ -23,13 +23,19 @@ def _tree(r):
             np.asarray(span).reshape(-1).tolist())
 
 
-def _check(bundle, docs, label):
+def _check(bundle, docs, label, derived=(), invalid=()):
+    """GPU (single and batched) against the C restatement on every document; the documents at
+    `derived` are sentences of the grammar and must be accepted, those at `invalid` rejected."""
     from alpacc_amd.runtime import Context
     from oracle.c import coracle
     orc = coracle.Oracle(bundle)
     ctx = Context(bundle)
     try:
         want = [_tree(orc.parse(d)) for d in docs]
+        for i in derived:
+            assert want[i] is not None, (label, i, "derived sentence rejected")
+        for i in invalid:
+            assert want[i] is None, (label, i, "invalid document accepted")
         for i, d in enumerate(docs):
             assert _tree(ctx.parse_int(d)) == want[i], (label, i, len(d))
         got = ctx.parse_int_batch(docs)
@@ -48,7 +54,9 @@ def test_random_grammars(part):
         for target in (1, 4, 16, 60, 1000):
             d = sentence(prods, seed * 100 + target, target)
             docs += [d, corrupt(d, seed + target)]
-        accepted += _check(b, docs, (seed, q, k))
+        # derived sentences must be accepted, the empty one aside (the lexer rejects n = 0)
+        accepted += _check(b, docs, (seed, q, k),
+                           derived=[i for i in range(0, len(docs), 2) if docs[i]])
     print(f"part {part}: {accepted} accepted", flush=True)
     assert accepted > 0
 
@@ -62,7 +70,7 @@ def test_random_grammars_large():
         if len(sentence(prods, seed, 2000)) < 1000:
             continue
         d = sentence(prods, seed + 7, 60_000)
-        assert _check(b, [d, corrupt(d, seed)], (seed, q, k, "large")) >= 1
+        assert _check(b, [d, corrupt(d, seed)], (seed, q, k, "large"), derived=[0]) >= 1
         print(f"grammar {seed} q={q} k={k}: {len(d)} bytes", flush=True)
         done += 1
     assert done == 6
@@ -100,4 +108,4 @@ def test_reference_llp_grammars(g):
     from alpacc_amd.gen.tables import generate
     b = generate(os.path.join(GRAMMARS, f"{g}.alp"))
     docs = _reference_docs(g, np.random.default_rng(3))
-    assert _check(b, docs, g) >= 6
+    assert _check(b, docs, g, derived=range(6), invalid=range(6, 8)) == 6
