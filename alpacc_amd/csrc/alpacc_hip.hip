@@ -2,16 +2,23 @@
 //
 // Device path for the runtime of the reference (futhark/lexer.fut, futhark/parser.fut):
 //
-//   lexer : one persistent kernel. Every wavefront owns a 2 KiB tile (64 lanes x 32 bytes) and
-//           runs the whole pipeline for it: byte classes, a serial endomorphism walk per lane
-//           with the (endo x class) table in LDS, a wave64 Kogge-Stone scan of the lane
-//           endomorphisms with the (endo x endo) table, a decoupled look-back across tiles,
-//           then two walks of the plain lexer DFA from the now known lane start states: one to
-//           count token ends and token starts, one (after a second look-back over the counts)
-//           to write (kind, start, end) of every token. Input bytes are read from HBM once.
-//   parser: pair-table lookup per token window, segmented copy of bracket / production spans,
-//           bracket depth scan, bracket matching and parents through a 64-ary min-tree
-//           "previous smaller-or-equal" search, terminal-node substitution.
+//   lexer : three phases, the input read twice.
+//           lex_reduce - every wavefront owns a 2 KiB tile (64 lanes x 32 bytes); each lane walks
+//             its bytes through the (endomorphism x byte class) table in LDS, and a wave64
+//             Blelloch scan composes the 64 lane endomorphisms with the (endo x endo) table.
+//           lex_scan_* / lex_tile_states - exclusive composition of the tile endomorphisms and
+//             the DFA state at every tile start (n / 2048 elements).
+//           lex_emit - lanes walk the plain lexer DFA from their known start states once,
+//             scan (token count, last token start) across the block and, by a decoupled
+//             look-back, across blocks, and write (kind, start, end) of every non-ignored token
+//             through LDS staging, coalesced.
+//   parser: one scan at the level of key positions. parse_keys2 maps every window of q + k token
+//           kinds to its pair entry; pscan_* scan the entries' totals (brackets, productions,
+//           depth, lowest depth, sum of arity - 1); parse_expand2 writes every position's
+//           brackets and productions (load-balanced, with two levels of adjacent bracket
+//           cancellation) and the tree's node and span words; bracket_match and parse_parents
+//           run a wave-cooperative "previous smaller-or-equal" search over 64-element chunks and
+//           a 64-ary min hierarchy.
 //
 // All arithmetic is integer. Tables are produced by alpacc_amd/gen (the restated generator).
 
