@@ -28,6 +28,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2303,6 +2304,7 @@ struct alpacc_ctx {
   size_t nstackw = 0, nprodw = 0;
   // scratch (grow-only)
   std::vector<std::pair<void *, size_t>> scratch;
+  void *noalloc_scratch = nullptr;  // device address of this device's allocation-free scratch
   uint32_t *h_small = nullptr;  // pinned host words for results
   // profiling: HIP events on the call's stream around each stage
   bool prof = false;
@@ -2470,13 +2472,89 @@ int fill_levels(alpacc_ctx *ctx, const Levels &la, const Levels &lb, hipStream_t
   return 0;
 }
 
+// The lexer's scratch for one unchunked call over n bytes: result words, the lane-exclusive
+// endomorphisms, the tile and scan-block endomorphisms, the tile start states and the look-back
+// words of the count scan.
+struct LexScratch {
+  uint32_t *small;       // 16 words: [0] count-tile counter, [4..6] accept, tokens, time-out
+  uint16_t *lane_excl;   // [tiles * 64]
+  uint16_t *tile_agg;    // [tiles]
+  uint16_t *tile_excl;   // [tiles]
+  uint16_t *block_agg;   // [scan blocks]
+  uint8_t *tile_state;   // [tiles]
+  uint64_t *st_count;    // [count tiles]
+};
+
+constexpr uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
+
+// Bytes of a LexScratch carved out of one buffer (every part 256-byte aligned), and the carving.
+constexpr uint64_t lex_scratch_bytes(uint64_t n) {
+  return align256(64) +
+         align256(((n + LEX_TILE - 1) / LEX_TILE) * WAVE * 2) +
+         2 * align256(((n + LEX_TILE - 1) / LEX_TILE) * 2) +
+         align256((((n + LEX_TILE - 1) / LEX_TILE + LEX_SCAN_BLOCK - 1) / LEX_SCAN_BLOCK) * 2) +
+         align256((n + LEX_TILE - 1) / LEX_TILE) +
+         align256((((n + LEX_TILE - 1) / LEX_TILE + LEX_CT_WAVES - 1) / LEX_CT_WAVES) * 8);
+}
+
+void lex_scratch_carve(uint8_t *base, uint64_t n, LexScratch &s) {
+  const uint64_t tiles = (n + LEX_TILE - 1) / LEX_TILE;
+  const uint64_t ctiles = (tiles + LEX_CT_WAVES - 1) / LEX_CT_WAVES;
+  const uint64_t sblocks = (tiles + LEX_SCAN_BLOCK - 1) / LEX_SCAN_BLOCK;
+  uint64_t off = 0;
+  auto take = [&](uint64_t bytes) {
+    uint8_t *p = base + off;
+    off += align256(bytes);
+    return p;
+  };
+  s.small = (uint32_t *)take(64);
+  s.lane_excl = (uint16_t *)take(tiles * WAVE * 2);
+  s.tile_agg = (uint16_t *)take(tiles * 2);
+  s.tile_excl = (uint16_t *)take(tiles * 2);
+  s.block_agg = (uint16_t *)take(sblocks * 2);
+  s.tile_state = take(tiles);
+  s.st_count = (uint64_t *)take(ctiles * 8);
+}
+
+}  // namespace
+
+// The allocation-free lexer entry's scratch: one array per GPU, held until the process ends, for
+// documents of up to NOALLOC_MAX_BYTES. Every alpacc_lex_noalloc call on a GPU uses that GPU's copy
+// (calls that overlap on one GPU race on it). Host code never passes the array's name to a kernel
+// or a copy: it is a host-side stand-in there. The device address comes from hipGetSymbolAddress,
+// taken once per device (noalloc_scratch_setup, called by alpacc_create with the device current).
+constexpr uint64_t NOALLOC_MAX_BYTES = 64ull << 20;
+constexpr uint64_t NOALLOC_SCRATCH_BYTES = lex_scratch_bytes(NOALLOC_MAX_BYTES);
+static_assert(NOALLOC_MAX_BYTES == ALPACC_NOALLOC_MAX_BYTES, "limit differs from the header's");
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device` (the current device); nullptr on failure.
+void *noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    g_noalloc_ptr[device] = ptr;
+  }
+  return g_noalloc_ptr[device];
+}
+
 // The lexer. chunk == 0: unchunked (kinds/starts/ends as u32). chunk > 0: the reference's chunk
 // loop with chunks of `chunk` bytes (kinds, starts64/ends64).
 // sync = false (unchunked only): return ALPACC_ACCEPT once the kernels are queued; the verdict,
 // the token count and the time-out flag stay in the device words result[0..2] (B_SMALL + 4).
+// fixed (unchunked only): scratch carved from a caller-provided buffer; nothing is allocated.
 int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, uint32_t *starts,
             uint32_t *ends, uint64_t *num_tokens, hipStream_t st, uint64_t chunk = 0,
-            uint64_t *starts64 = nullptr, uint64_t *ends64 = nullptr, bool sync = true) {
+            uint64_t *starts64 = nullptr, uint64_t *ends64 = nullptr, bool sync = true,
+            const LexScratch *fixed = nullptr) {
   if (!ctx->has_lexer) {
     ctx->err = "bundle has no lexer";
     return ALPACC_ERR_ARG;
@@ -2494,6 +2572,10 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   }
   if (n >= (1ull << 31) - 2) {
     ctx->err = "documents must be shorter than 2^31 - 2 bytes";
+    return ALPACC_ERR_ARG;
+  }
+  if (fixed && chunk > 0) {
+    ctx->err = "the chunked lexer has no fixed-scratch form";
     return ALPACC_ERR_ARG;
   }
   if (chunk > 0) {
@@ -2517,13 +2599,23 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   const uint64_t tiles = (n + LEX_TILE - 1) / LEX_TILE;
   const uint64_t ctiles = (tiles + LEX_CT_WAVES - 1) / LEX_CT_WAVES;
   const uint64_t sblocks = (tiles + LEX_SCAN_BLOCK - 1) / LEX_SCAN_BLOCK;
-  uint32_t *small = (uint32_t *)ctx->buf(B_SMALL, 64);
-  uint16_t *lane_excl = (uint16_t *)ctx->buf(B_LANE_EXCL, tiles * WAVE * sizeof(uint16_t));
-  uint16_t *tile_agg = (uint16_t *)ctx->buf(B_TILE_AGG, tiles * sizeof(uint16_t));
-  uint16_t *tile_excl = (uint16_t *)ctx->buf(B_TILE_EXCL, tiles * sizeof(uint16_t));
-  uint16_t *block_agg = (uint16_t *)ctx->buf(B_BLOCK_AGG, sblocks * sizeof(uint16_t));
-  uint8_t *tile_state = (uint8_t *)ctx->buf(B_TILE_STATE, tiles);
-  uint64_t *st_count = (uint64_t *)ctx->buf(B_ST_COUNT, ctiles * sizeof(uint64_t));
+  LexScratch sc;
+  if (fixed) {
+    sc = *fixed;
+  } else {
+    sc.small = (uint32_t *)ctx->buf(B_SMALL, 64);
+    sc.lane_excl = (uint16_t *)ctx->buf(B_LANE_EXCL, tiles * WAVE * sizeof(uint16_t));
+    sc.tile_agg = (uint16_t *)ctx->buf(B_TILE_AGG, tiles * sizeof(uint16_t));
+    sc.tile_excl = (uint16_t *)ctx->buf(B_TILE_EXCL, tiles * sizeof(uint16_t));
+    sc.block_agg = (uint16_t *)ctx->buf(B_BLOCK_AGG, sblocks * sizeof(uint16_t));
+    sc.tile_state = (uint8_t *)ctx->buf(B_TILE_STATE, tiles);
+    sc.st_count = (uint64_t *)ctx->buf(B_ST_COUNT, ctiles * sizeof(uint64_t));
+  }
+  uint32_t *small = sc.small;
+  uint16_t *lane_excl = sc.lane_excl, *tile_agg = sc.tile_agg, *tile_excl = sc.tile_excl;
+  uint16_t *block_agg = sc.block_agg;
+  uint8_t *tile_state = sc.tile_state;
+  uint64_t *st_count = sc.st_count;
   if (!small || !lane_excl || !tile_agg || !tile_excl || !block_agg || !tile_state || !st_count) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
@@ -3203,6 +3295,8 @@ alpacc_ctx *alpacc_create(const alpacc_tables *t, int device) {
   }
   alpacc_ctx *ctx = new alpacc_ctx();
   ctx->device = device;
+  // the allocation-free entry's scratch address on this device (nullptr: that entry reports it)
+  ctx->noalloc_scratch = noalloc_scratch_setup(device);
   (void)hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device);
   std::string &err = g_create_error;
   if (hipHostMalloc((void **)&ctx->h_small, 256, hipHostMallocDefault) != hipSuccess) {
@@ -3356,6 +3450,33 @@ int alpacc_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_kin
   ctx->err.clear();
   (void)hipSetDevice(ctx->device);
   const int rc = run_lex(ctx, d_in, n, d_kinds, d_starts, d_ends, num_tokens, (hipStream_t)stream);
+  if (ctx->prof) ctx->prof_collect();
+  return rc;
+}
+
+int alpacc_lex_noalloc(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_kinds,
+                       uint32_t *d_starts, uint32_t *d_ends, uint64_t cap, uint64_t *num_tokens,
+                       void *stream) {
+  if (!ctx || !num_tokens || (n > 0 && (!d_in || !d_kinds || !d_starts || !d_ends))) return ALPACC_ERR_ARG;
+  ctx->launches = 0;
+  ctx->err.clear();
+  if (n > NOALLOC_MAX_BYTES) {
+    ctx->err = "alpacc_lex_noalloc takes documents of at most ALPACC_NOALLOC_MAX_BYTES (64 MiB)";
+    return ALPACC_ERR_ARG;
+  }
+  if (cap < n) {
+    ctx->err = "token capacity must be at least the input length";
+    return ALPACC_ERR_CAPACITY;
+  }
+  if (!ctx->noalloc_scratch) {
+    ctx->err = "no address for the allocation-free scratch on this device";
+    return ALPACC_ERR_HIP;
+  }
+  (void)hipSetDevice(ctx->device);
+  LexScratch sc;
+  lex_scratch_carve((uint8_t *)ctx->noalloc_scratch, n, sc);
+  const int rc = run_lex(ctx, d_in, n, d_kinds, d_starts, d_ends, num_tokens, (hipStream_t)stream,
+                         0, nullptr, nullptr, true, &sc);
   if (ctx->prof) ctx->prof_collect();
   return rc;
 }

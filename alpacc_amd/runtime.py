@@ -63,7 +63,7 @@ EXPORTS = ("alpacc_create", "alpacc_create_error", "alpacc_destroy", "alpacc_las
            "alpacc_event_elapsed_ms", "alpacc_profile_enable", "alpacc_profile_read",
            "alpacc_lex_chunked", "alpacc_parse_chunked", "alpacc_lex_batch", "alpacc_parse_batch",
            "alpacc_last_counts", "alpacc_host_alloc", "alpacc_host_free", "alpacc_copy_device",
-           "alpacc_test", "alpacc_free", "alpacc_test_force_timeout")
+           "alpacc_test", "alpacc_free", "alpacc_test_force_timeout", "alpacc_lex_noalloc")
 
 # the chunk size of the reference's generated entries (Generator.hs:24-76: lexer.lex_int 16777216)
 REFERENCE_CHUNK = 16777216
@@ -91,6 +91,7 @@ def load_library():
     L.alpacc_last_launch_count.argtypes = [vp]
     L.alpacc_test_force_timeout.argtypes = [vp, ctypes.c_int]
     L.alpacc_lex.argtypes = [vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
+    L.alpacc_lex_noalloc.argtypes = [vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
     L.alpacc_parse.argtypes = [vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
     L.alpacc_parse_tokens.argtypes = [vp, vp, vp, vp, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
     L.alpacc_lex_chunked.argtypes = [vp, vp, u64, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]

@@ -85,6 +85,19 @@ int alpacc_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_kin
                uint32_t *d_starts, uint32_t *d_ends, uint64_t cap, uint64_t *num_tokens,
                void *stream);
 
+/* Allocation-free form of alpacc_lex, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It has no counterpart
+ * in the reference. Same arguments, semantics and results as alpacc_lex, for documents of at most
+ * ALPACC_NOALLOC_MAX_BYTES; larger ones return ALPACC_ERR_ARG before any GPU work. Its scratch
+ * (about 4.3 MiB) is one static device array per GPU, held until the process ends, so the call
+ * allocates nothing (no hipMalloc, hipFree or allocating library routine). Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams, other threads, other contexts)
+ * race and must be serialised by the caller. */
+#define ALPACC_NOALLOC_MAX_BYTES (64ull << 20)
+int alpacc_lex_noalloc(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_kinds,
+                       uint32_t *d_starts, uint32_t *d_ends, uint64_t cap, uint64_t *num_tokens,
+                       void *stream);
+
 /* parse_int of a lexer+parser artifact (Generator.hs:35-44: lex_int then parser.parse_int,
  * futhark/parser.fut:316-336). Outputs device u32 arrays: parent[cap], node[cap], span[2*cap]
  * (start, end; zeros for production nodes). If cap is too small the call returns
