@@ -41,6 +41,8 @@ def load():
                                    ctypes.POINTER(vp), ctypes.POINTER(vp), _i64p]
         L.oracle_free.argtypes = [vp]
         L.oracle_threads.restype = ctypes.c_int
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_set_threads.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -130,9 +132,16 @@ class Oracle:
         return (_take(self.lib, p, np.uint32, N), _take(self.lib, nd, np.uint32, N),
                 _take(self.lib, sp, np.uint32, 2 * N))
 
-    def parse(self, data):
-        """None when rejected, else (parent u32, node u32, span u32[2N]) like alpacc_parse."""
-        r = self.lex(data, wide=True)
+    def set_threads(self, n: int) -> int:
+        """OpenMP threads of the following calls (process-wide); returns the count now in use."""
+        self.threads = self.lib.oracle_set_threads(int(n))
+        return self.threads
+
+    def parse(self, data, chunk: int = 0):
+        """None when rejected, else (parent u32, node u32, span u32[2N]) like alpacc_parse.
+        chunk > 0: the lexer's chunk loop (lexer.fut:144-172) over chunks of that many bytes, as
+        the generated parse_int runs it with 16 MiB chunks."""
+        r = self.lex(data, chunk=chunk, wide=True)
         if r is None:
             return None
         return self.parse_tokens(*r)

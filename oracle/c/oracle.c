@@ -53,6 +53,12 @@ static inline uint32_t compose(const otables *t, uint32_t a, uint32_t b) {
 
 int oracle_threads(void) { return omp_get_max_threads(); }
 
+/* threads of the following calls (the 1-thread CPU baseline); returns the new count */
+int oracle_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
+
 /* chunk bounds for thread p of P over [0, n) */
 static inline void chunk_of(int64_t n, int p, int P, int64_t *lo, int64_t *hi) {
   *lo = n * p / P;
