@@ -128,6 +128,7 @@ struct LexArgs {
   const uint8_t *in;
   uint64_t n;
   uint32_t *kinds;
+  uint8_t *kinds8;            // when set (unchunked): token kinds as bytes here instead of kinds
   uint32_t *starts;
   uint32_t *ends;
   const uint8_t *byte_class;
@@ -292,21 +293,25 @@ __device__ __forceinline__ void load_classes(const uint8_t *in, bool aligned, co
 }
 
 // K1: per wave tile, the endomorphism of every lane's bytes (serial walk through the
-// (endo x class) table in LDS) and their wave-exclusive composition (Blelloch up/down sweep with
-// the (endo x endo) table, 126 lookups per wave). Persistent, grid-stride over wave tiles.
-template <bool COMP_IN_LDS, bool BATCH>
+// (endo x class) table) and their wave-exclusive composition (Blelloch up/down sweep with the
+// (endo x endo) table, 126 lookups per wave). Persistent, grid-stride over wave tiles.
+// TAB: where the tables sit: 2 = both in LDS, 1 = the (endo x class) table in LDS and the
+// composition table in L2, 0 = both in L2 (an (endo x class) table larger than LDS).
+template <int TAB, bool BATCH>
 __global__ void __launch_bounds__(LEX_BLOCK) lex_reduce(LexArgs p) {
   extern __shared__ __align__(16) uint8_t smem[];
   const uint32_t e = p.e, C = p.C;
   uint16_t *s_step = reinterpret_cast<uint16_t *>(smem);
-  uint8_t *s_cls = reinterpret_cast<uint8_t *>(s_step + e * C);
+  uint8_t *s_cls = TAB >= 1 ? reinterpret_cast<uint8_t *>(s_step + e * C) : smem;
   uint16_t *s_comp = reinterpret_cast<uint16_t *>(s_cls + 256);
-  for (uint32_t i = threadIdx.x; i < e * C; i += blockDim.x) s_step[i] = p.endo_step[i];
+  if (TAB >= 1)
+    for (uint32_t i = threadIdx.x; i < e * C; i += blockDim.x) s_step[i] = p.endo_step[i];
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_cls[i] = p.byte_class[i];
-  if (COMP_IN_LDS)
+  if (TAB == 2)
     for (uint32_t i = threadIdx.x; i < e * e; i += blockDim.x) s_comp[i] = p.endo_comp[i];
   __syncthreads();
-  const uint16_t *comp_tab = COMP_IN_LDS ? s_comp : p.endo_comp;
+  const uint16_t *comp_tab = TAB == 2 ? s_comp : p.endo_comp;
+  const uint16_t *step_tab = TAB >= 1 ? s_step : p.endo_step;
   const int lane = threadIdx.x & (WAVE - 1);
   const uint32_t wpb = blockDim.x / WAVE;
   const uint32_t stride = gridDim.x * wpb;
@@ -327,11 +332,11 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_reduce(LexArgs p) {
       uint32_t x = 0;
       if (valid == LEX_IPT) {
 #pragma unroll
-        for (int j = 0; j < LEX_IPT; ++j) x = s_step[__umul24(x, C) + LEX_CLS(j)];
+        for (int j = 0; j < LEX_IPT; ++j) x = step_tab[__umul24(x, C) + LEX_CLS(j)];
       } else {
 #pragma unroll
         for (int j = 0; j < LEX_IPT; ++j)
-          if (j < valid) x = s_step[x * C + LEX_CLS(j)];
+          if (j < valid) x = step_tab[x * C + LEX_CLS(j)];
       }
       v[h] = x;
     }
@@ -569,7 +574,8 @@ __device__ __forceinline__ uint64_t pack_count(uint32_t cnt, int32_t last, uint6
 // chunked mode; the last element of every chunk but the last ends no token (i != n - 1 in
 // lex_step, lexer.fut:104-107), spans are shifted as lex_step does, and every chunk's last
 // element is recorded in chunk_info for lex_chunk_fixup.
-template <bool CHUNKED, bool BATCH>
+// DFA_LDS: the DFA table ((m + 1) x C entries) is copied to LDS; else it is read from L2.
+template <bool CHUNKED, bool BATCH, bool DFA_LDS>
 __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   __shared__ uint16_t s_dfa[LEX_MAX_DFA];
   __shared__ uint8_t s_cls[256];
@@ -582,7 +588,9 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   __shared__ uint32_t s_stage[LEX_CT_WAVES][2][LEX_STAGE];  // starts, ends
   __shared__ uint8_t s_kind[LEX_CT_WAVES][LEX_STAGE];
   const uint32_t C = p.C;
-  for (uint32_t i = threadIdx.x; i < p.m1 * C; i += blockDim.x) s_dfa[i] = p.dfa_step[i];
+  if (DFA_LDS)
+    for (uint32_t i = threadIdx.x; i < p.m1 * C; i += blockDim.x) s_dfa[i] = p.dfa_step[i];
+  const uint16_t *dfa = DFA_LDS ? s_dfa : p.dfa_step;
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_cls[i] = p.byte_class[i];
   if (threadIdx.x == 0) s_ctile = atomicAdd(p.tile_counter, 1u);
   __syncthreads();
@@ -628,8 +636,8 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   uint32_t lab_end = 0;
   if (valid == LEX_IPT && base + LEX_IPT < n) {
     // a full lane followed by another byte (every lane but a document's last): no bounds in the
-    // loop. State x class indexes fit 24 bits (the table has at most LEX_MAX_DFA entries).
-    uint32_t ent = s_dfa[__umul24(s_lane, C) + LEX_CLS(0)];
+    // loop. State x class indexes fit 24 bits (at most 256 states and 256 classes).
+    uint32_t ent = dfa[__umul24(s_lane, C) + LEX_CLS(0)];
     uint32_t s = ent & 0xFFu;
     uint32_t lab = ent >> 9;
     bool lab_kept = lab != ign;
@@ -638,7 +646,7 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
     for (int j = 0; j < LEX_IPT; ++j) {
       labs[j >> 2] |= (lab & 0xFFu) << (8 * (j & 3));
       const uint32_t cn = j + 1 < LEX_IPT ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
-      const uint32_t ent2 = s_dfa[__umul24(s, C) + cn];
+      const uint32_t ent2 = dfa[__umul24(s, C) + cn];
       const bool prod2 = (ent2 >> 8) & 1u;
       if (prod2 && lab_kept) endm |= 1u << j;
       if (j + 1 < LEX_IPT) {
@@ -652,7 +660,7 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
     s_end = s;
     lab_end = lab;
   } else if (valid > 0) {
-    uint32_t ent = s_dfa[s_lane * C + LEX_CLS(0)];
+    uint32_t ent = dfa[s_lane * C + LEX_CLS(0)];
     uint32_t s = ent & 0xFFu;
     uint32_t lab = ent >> 9;
     if (((ent >> 8) & 1u) && lab != ign) startm |= 1u;
@@ -663,7 +671,7 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
         const bool inner = j + 1 < valid;
         const bool has_next = inner || (base + j + 1 < n);
         const uint32_t cn = inner ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
-        const uint32_t ent2 = has_next ? s_dfa[s * C + cn] : 0u;
+        const uint32_t ent2 = has_next ? dfa[s * C + cn] : 0u;
         const bool prod2 = (ent2 >> 8) & 1u;
         const uint32_t lab2 = ent2 >> 9;
         if (has_next && prod2 && lab != ign) endm |= 1u << j;
@@ -831,7 +839,8 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
       const uint64_t g = (uint64_t)wout + r0 + i;
       uint32_t sv = sts[i];
       if (sv == START_PENDING) sv = pending_start;
-      p.kinds[g] = stk[i];
+      if (!CHUNKED && p.kinds8) p.kinds8[g] = stk[i];
+      else p.kinds[g] = stk[i];
       if (CHUNKED) {
         uint64_t cs, ce;
         chunk_span(p, sv, ste[i], cs, ce);
@@ -900,7 +909,8 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
         total = out + 1;
       }
     } else if (lab_end != ign) {
-      p.kinds[out] = lab_end;
+      if (p.kinds8) p.kinds8[out] = (uint8_t)lab_end;
+      else p.kinds[out] = lab_end;
       p.starts[out] = (uint32_t)cur;
       p.ends[out] = (uint32_t)n;
       total = out + 1;
@@ -1356,6 +1366,7 @@ __device__ __forceinline__ void pk_cancel2(const int32_t (&n1)[PK_IPT],
 
 struct PArgs {
   const uint32_t *kinds;
+  const uint8_t *kinds8;  // when set: the token kinds as bytes (straight from the lexer)
   uint64_t T;  // tokens
   uint64_t P;  // positions: T + 2 (start and end terminal added, to_keys parser.fut:235-240)
   // single document straight after the lexer: the token count on the device (T, P and the tile
@@ -1365,6 +1376,10 @@ struct PArgs {
   const uint8_t *index8;    // dense pair index, entry or 0xFF (when E < 255)
   const uint32_t *index32;  // dense pair index, entry or 0xFFFFFFFF
   uint32_t index_len;
+  // hashed pair table (index32 NULL): dense key index per slot, entry id per slot
+  const uint64_t *hkeys;
+  const uint32_t *hvals;
+  uint32_t hbits, hprobe;   // log2 of the slots; the longest probe sequence of any stored key
   const EntInfo *einfo;
   uint32_t E;
   const uint32_t *stackw;  // bracket symbol (| 0x8000 left) | depth value relative to the entry << 16
@@ -1386,7 +1401,19 @@ struct PArgs {
   uint32_t ndocs;
   uint32_t *doc_flags;       // [ndocs] bit 0: missing pair, bit 1: bracket mismatch
   uint32_t *chunk_doc;       // [N / 64] document of the first production of every 64-chunk
+  // batched mode: capacities of the bracket and production scratch, sized before the launch; a
+  // batch whose totals exceed them (or 2^31) writes no elements (the host then sizes them again)
+  uint64_t bcap, ncap;
 };
+
+// Batched mode: the brackets / productions the element kernels process: the grand totals, or
+// none when they exceed the scratch capacities (then nothing is written).
+__device__ __forceinline__ bool batch_elems_fit(const PArgs &a) {
+  const PTup t = *a.total;
+  return a.flags[1] == 0u && t.nb >= 0 && t.np >= 0 && (uint64_t)t.nb <= a.bcap &&
+         (uint64_t)t.np <= a.ncap;
+}
+
 
 struct PDoc {
   uint64_t tok_base;    // first token slot of the document (batched lexer layout)
@@ -1405,6 +1432,7 @@ struct PDoc {
 // position inside the document.
 struct PosView {
   const uint32_t *kinds;
+  const uint8_t *kinds8;
   uint64_t P;
   int64_t p0;
   uint64_t tok_base;
@@ -1418,6 +1446,7 @@ __device__ __forceinline__ PosView pos_view(const PArgs &a, uint64_t tile) {
   PosView v;
   if (!BATCH) {
     v.kinds = a.kinds;
+    v.kinds8 = a.kinds8;
     v.P = a.T_dev ? std::min<uint64_t>(*a.T_dev, a.T) + 2 : a.P;
     v.p0 = (int64_t)(tile * PK_TILE);
     v.tok_base = 0;
@@ -1429,6 +1458,7 @@ __device__ __forceinline__ PosView pos_view(const PArgs &a, uint64_t tile) {
     const PDoc &doc = a.pdocs[d];
     const uint32_t lt = (uint32_t)tile - doc.ptile_base;
     v.kinds = a.kinds + doc.tok_base;
+    v.kinds8 = a.kinds8 ? a.kinds8 + doc.tok_base : nullptr;
     v.P = (uint64_t)doc.T + 2;
     v.p0 = (int64_t)lt * PK_TILE;
     v.tok_base = doc.tok_base;
@@ -1449,12 +1479,11 @@ __device__ __forceinline__ uint64_t pk_ntiles(const PArgs &a) {
 
 // symbol of position x of [start] ++ kinds ++ [end], the empty terminal outside (get_key,
 // parser.fut:117-132); a kind the table does not know rejects
-__device__ __forceinline__ uint32_t pk_symbol(const PArgs &a, const uint32_t *kinds, uint64_t P,
-                                              int64_t x) {
-  if (x < 0 || x >= (int64_t)P) return a.empty_t;
+__device__ __forceinline__ uint32_t pk_symbol(const PArgs &a, const PosView &pv, int64_t x) {
+  if (x < 0 || x >= (int64_t)pv.P) return a.empty_t;
   if (x == 0) return a.start_t;
-  if (x == (int64_t)P - 1) return a.end_t;
-  return kinds[x - 1];
+  if (x == (int64_t)pv.P - 1) return a.end_t;
+  return pv.kinds8 ? (uint32_t)pv.kinds8[x - 1] : pv.kinds[x - 1];
 }
 
 // keys and pair lookups (keys / lookup / to_keys, parser.fut:110-151, 235-240); per-tile totals.
@@ -1462,6 +1491,21 @@ __device__ __forceinline__ uint32_t pk_symbol(const PArgs &a, const uint32_t *ki
 // tile; the u8 pair index is loaded into LDS once per block.
 constexpr int PKK_THREADS = 1024;
 constexpr int PKK_TILES = PKK_THREADS / PK_THREADS;
+
+// Entry id of the key window with dense index di in the hashed pair table, or 0xFFFFFFFF: linear
+// probing from the multiplicative-hash home slot, at most hprobe slots (every stored key lies
+// within them, counted when the tables were built), stopping at a free slot.
+__device__ __forceinline__ uint32_t pk_hash_lookup(const PArgs &a, uint64_t di) {
+  const uint64_t mask = (1ull << a.hbits) - 1;
+  uint64_t h = (di * 0x9E3779B97F4A7C15ull) >> (64 - a.hbits);
+  for (uint32_t i = 0; i < a.hprobe; ++i) {
+    const uint64_t key = a.hkeys[h];
+    if (key == di) return a.hvals[h];
+    if (key == ~0ull) break;
+    h = (h + 1) & mask;
+  }
+  return 0xFFFFFFFFu;
+}
 
 // PosView of tile `tile` (an empty view past the tile count) and the symbols of the key windows of
 // this thread's PK_IPT positions in it.
@@ -1472,12 +1516,12 @@ __device__ __forceinline__ void keys_window(const PArgs &a, uint64_t tile, uint6
   if (tile < ntiles) {
     pv = pos_view<BATCH>(a, tile);
   } else {
-    pv = PosView{a.kinds, 0, 0, 0, 0, false, 0};  // every field set: the view stays in registers
+    pv = PosView{a.kinds, a.kinds8, 0, 0, 0, 0, false, 0};  // every field set: the view stays in registers
   }
   const int64_t lp0 = pv.p0 + (int64_t)(threadIdx.x % PK_THREADS) * PK_IPT;  // in the document
 #pragma unroll
   for (int j = 0; j < PK_IPT + KW - 1; ++j)
-    sym[j] = j < PK_IPT + (int)W - 1 ? pk_symbol(a, pv.kinds, pv.P, lp0 - (int64_t)a.q + j) : 0u;
+    sym[j] = j < PK_IPT + (int)W - 1 ? pk_symbol(a, pv, lp0 - (int64_t)a.q + j) : 0u;
 }
 
 // KW: the largest key window q + k the instance handles (4 or PK_MAXW); the next iteration's
@@ -1544,7 +1588,7 @@ parse_keys2(PArgs a) {
         id = unknown ? 0xFFu : pk_smem[di];
         if (id == 0xFFu) id = 0xFFFFFFFFu;
       } else {
-        id = unknown ? 0xFFFFFFFFu : a.index32[di];
+        id = unknown ? 0xFFFFFFFFu : (a.index32 ? a.index32[di] : pk_hash_lookup(a, di));
       }
       if (id == 0xFFFFFFFFu) {
         wmiss = true;
@@ -1849,8 +1893,11 @@ constexpr size_t PX_TABLE_LDS_MAX = 32 << 10;  // stackw + prodw + entry infos, 
 // expansion: one lane per element, whatever the positions' element counts). Chunks of 64 elements
 // shared with a neighbouring wavefront take their minimum through atomicMin on an array cleared to
 // INT32_MAX.
+// Five workgroups of 4 waves per CU (84 VGPRs): without the hint the compiler settles at 106
+// VGPRs (4 waves per SIMD) for the batched and device-sized forms.
 template <typename S, bool BATCH>
-__global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *starts, const S *ends,
+__global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu(5)))
+parse_expand2(PArgs a, const S *starts, const S *ends,
                                                             int32_t *depth, uint16_t *br,
                                                             int32_t *dl1, int32_t *sx, int32_t *sl1,
                                                             uint32_t *node, S *span,
@@ -1868,8 +1915,12 @@ __global__ void __launch_bounds__(PK_THREADS) parse_expand2(PArgs a, const S *st
   uint32_t *s_can = reinterpret_cast<uint32_t *>(s_einfo + a.E);
   const int lane = threadIdx.x & (WAVE - 1);
   const int w = threadIdx.x / WAVE;
+  // one block per position tile. The batched grid covers a bound of the tile count (the count is
+  // known on the device only); a batch that does not fit its element scratch writes nothing.
   const uint64_t tile = blockIdx.x;
-  if (BATCH && tile >= *a.ntiles_dev) return;  // the grid covers the largest possible tile count
+  if (BATCH) {
+    if (tile >= *a.ntiles_dev || !batch_elems_fit(a)) return;
+  }
   const PosView pv = pos_view<BATCH>(a, tile);
   const int subst = pv.subst;
   const uint64_t pbase = tile * PK_TILE + (uint64_t)threadIdx.x * PK_IPT;  // slot
@@ -2143,6 +2194,7 @@ template <bool BATCH>
 __device__ void bracket_match_run(const uint16_t *br, uint64_t B, const Levels &lv, uint32_t *flags,
                                   const PArgs &a, uint64_t blk, uint64_t nblk) {
   const int lane = threadIdx.x & (WAVE - 1);
+  if (BATCH) B = batch_elems_fit(a) ? std::min<uint64_t>(B, (uint32_t)a.total->nb) : 0;
   const uint64_t nch = (B + WAVE - 1) / WAVE;
   const uint64_t wstride = nblk * (blockDim.x / WAVE);
   bool bad = false;
@@ -2204,6 +2256,7 @@ template <bool BATCH>
 __device__ void parents_run(uint64_t N, const Levels &lv, uint32_t *parent, const PArgs &a,
                             uint64_t blk, uint64_t nblk) {
   const int lane = threadIdx.x & (WAVE - 1);
+  if (BATCH) N = batch_elems_fit(a) ? std::min<uint64_t>(N, (uint32_t)a.total->np) : 0;
   const uint64_t nch = (N + WAVE - 1) / WAVE;
   const uint64_t wstride = nblk * (blockDim.x / WAVE);
   // one chunk per iteration: this search is bound by instruction issue, and a second chunk in
@@ -2234,22 +2287,57 @@ __device__ void parents_run(uint64_t N, const Levels &lv, uint32_t *parent, cons
   }
 }
 
-template <bool BATCH>
-__global__ void __launch_bounds__(256) bracket_match(const uint16_t *br, uint64_t B, Levels lv,
-                                                     uint32_t *flags, PArgs a) {
-  bracket_match_run<BATCH>(br, B, lv, flags, a, blockIdx.x, gridDim.x);
-}
-
-template <bool BATCH>
-__global__ void __launch_bounds__(256) parse_parents(uint64_t N, Levels lv, uint32_t *parent,
-                                                     PArgs a) {
-  parents_run<BATCH>(N, lv, parent, a, blockIdx.x, gridDim.x);
-}
-
 inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 16) {
   uint64_t g = (n + block - 1) / block;
   if (g == 0) g = 1;
   return (unsigned)std::min<uint64_t>(g, cap);
+}
+
+// lex_reduce / lex_emit with the table placement chosen at context creation
+template <bool BATCH>
+void launch_lex_reduce(int tab, unsigned blocks, size_t lds, const LexArgs &a, hipStream_t st) {
+  if (tab == 2) lex_reduce<2, BATCH><<<blocks, LEX_BLOCK, lds, st>>>(a);
+  else if (tab == 1) lex_reduce<1, BATCH><<<blocks, LEX_BLOCK, lds, st>>>(a);
+  else lex_reduce<0, BATCH><<<blocks, LEX_BLOCK, lds, st>>>(a);
+}
+
+template <bool CHUNKED, bool BATCH>
+void launch_lex_emit(bool dfa_lds, unsigned grid, const LexArgs &a, hipStream_t st) {
+  if (dfa_lds) lex_emit<CHUNKED, BATCH, true><<<grid, LEX_BLOCK, 0, st>>>(a);
+  else lex_emit<CHUNKED, BATCH, false><<<grid, LEX_BLOCK, 0, st>>>(a);
+}
+
+// BS: workgroup size (one 64-element chunk per wavefront and iteration, whatever the size)
+template <bool BATCH, int BS>
+__global__ void __launch_bounds__(BS) bracket_match(const uint16_t *br, uint64_t B, Levels lv,
+                                                    uint32_t *flags, PArgs a) {
+  bracket_match_run<BATCH>(br, B, lv, flags, a, blockIdx.x, gridDim.x);
+}
+
+template <bool BATCH, int BS>
+__global__ void __launch_bounds__(BS) parse_parents(uint64_t N, Levels lv, uint32_t *parent,
+                                                    PArgs a) {
+  parents_run<BATCH>(N, lv, parent, a, blockIdx.x, gridDim.x);
+}
+
+// The search kernels with workgroups of `bs` threads (256, 512 or 1024) over n elements, at most
+// `cap` workgroups of 256 threads' worth.
+template <bool BATCH>
+void launch_bracket_match(unsigned bs, unsigned cap, const uint16_t *br, uint64_t B,
+                          const Levels &lv, uint32_t *flags, const PArgs &a, hipStream_t st) {
+  const unsigned g = std::max(1u, grid_for(B, 256, cap) / (bs / 256));
+  if (bs == 1024) bracket_match<BATCH, 1024><<<g, 1024, 0, st>>>(br, B, lv, flags, a);
+  else if (bs == 512) bracket_match<BATCH, 512><<<g, 512, 0, st>>>(br, B, lv, flags, a);
+  else bracket_match<BATCH, 256><<<g, 256, 0, st>>>(br, B, lv, flags, a);
+}
+
+template <bool BATCH>
+void launch_parse_parents(unsigned bs, unsigned cap, uint64_t N, const Levels &lv,
+                          uint32_t *parent, const PArgs &a, hipStream_t st) {
+  const unsigned g = std::max(1u, grid_for(N, 256, cap) / (bs / 256));
+  if (bs == 1024) parse_parents<BATCH, 1024><<<g, 1024, 0, st>>>(N, lv, parent, a);
+  else if (bs == 512) parse_parents<BATCH, 512><<<g, 512, 0, st>>>(N, lv, parent, a);
+  else parse_parents<BATCH, 256><<<g, 256, 0, st>>>(N, lv, parent, a);
 }
 
 }  // namespace
@@ -2288,14 +2376,19 @@ struct alpacc_ctx {
   uint8_t *d_cls = nullptr;
   uint16_t *d_step = nullptr, *d_comp = nullptr, *d_dfa = nullptr;
   uint8_t *d_apply = nullptr, *d_accept = nullptr;
-  bool comp_in_lds = false;
-  size_t lex_lds = 0;
+  int lex_tab = 0;       // lex_reduce's table placement (see its TAB)
+  bool dfa_lds = true;   // lex_emit's DFA table in LDS
+  size_t lex_lds = 0;    // lex_reduce's dynamic LDS
+  int lex_tables_lds = 1;  // tuning: 0 keeps every lexer table in L2
   // parser tables
   bool has_parser = false;
   uint32_t q = 0, k = 0, nT = 0, start_t = 0, end_t = 0, empty_t = 0, E = 0;
   uint32_t *d_pair_index = nullptr;  // dense index, u32 entries
   uint8_t *d_index8 = nullptr;       // the same as u8 (E < 255 and small enough for LDS)
   size_t index_len = 0;
+  uint64_t *d_hkeys = nullptr;       // the hashed pair table (no dense index)
+  uint32_t *d_hvals = nullptr;
+  uint32_t hbits = 0, hprobe = 0;
   EntInfo *d_einfo = nullptr;
   uint32_t *d_stackw = nullptr;
   uint2 *d_prodw = nullptr;
@@ -2304,7 +2397,23 @@ struct alpacc_ctx {
   size_t nstackw = 0, nprodw = 0;
   // scratch (grow-only)
   std::vector<std::pair<void *, size_t>> scratch;
+  uint64_t table_bytes = 0;  // device bytes of the uploaded tables
   void *noalloc_scratch = nullptr;  // device address of this device's allocation-free scratch
+  // pinned host staging (grow-only) of the batched calls: descriptors in, results out
+  uint8_t *h_stage = nullptr;
+  size_t h_stage_len = 0;
+  // batched lex + parse: HIP graph of the last call's queue, replayed while the plan repeats
+  int graph_mode = 1;                 // 0 never, 1 from the second identical plan on, 2 always
+  hipStream_t cap_stream = nullptr;   // capture stream (default priority)
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint8_t> gplan;         // the plan gexec was captured from
+  std::vector<uint8_t> last_plan;     // the previous call's plan (graph_mode 1)
+  uint64_t graph_launches = 0, graph_captures = 0;
+  // brackets per input byte the batched scratch is sized for (grows with what calls need)
+  double b_per_byte = 0.25;
+  // launch shapes (alpacc_set_tuning): persistent lex_reduce / parse_keys2 blocks per CU, and the
+  // largest grid of the search kernels
+  uint32_t lex_bpc = 2, keys_bpc = 2, search_cap = 1u << 16, search_bs = 256;
   uint32_t *h_small = nullptr;  // pinned host words for results
   // profiling: HIP events on the call's stream around each stage
   bool prof = false;
@@ -2338,6 +2447,29 @@ struct alpacc_ctx {
       }
       rec[s] = false;
     }
+  }
+
+  // pinned staging of at least `bytes`; growing it first waits for the stream
+  uint8_t *stage(size_t bytes, hipStream_t st) {
+    if (h_stage_len < bytes) {
+      (void)hipStreamSynchronize(st);
+      if (h_stage) (void)hipHostFree(h_stage);
+      h_stage = nullptr;
+      h_stage_len = 0;
+      const size_t want = std::max<size_t>((bytes + 4095) & ~(size_t)4095, 64 << 10);
+      if (hipHostMalloc((void **)&h_stage, want, hipHostMallocDefault) != hipSuccess) {
+        h_stage = nullptr;
+        return nullptr;
+      }
+      h_stage_len = want;
+    }
+    return h_stage;
+  }
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    gplan.clear();
   }
 
   void *buf(int id, size_t bytes) {
@@ -2554,7 +2686,7 @@ void *noalloc_scratch_setup(int device) {
 int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, uint32_t *starts,
             uint32_t *ends, uint64_t *num_tokens, hipStream_t st, uint64_t chunk = 0,
             uint64_t *starts64 = nullptr, uint64_t *ends64 = nullptr, bool sync = true,
-            const LexScratch *fixed = nullptr) {
+            const LexScratch *fixed = nullptr, uint8_t *kinds8 = nullptr) {
   if (!ctx->has_lexer) {
     ctx->err = "bundle has no lexer";
     return ALPACC_ERR_ARG;
@@ -2627,6 +2759,7 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   a.in = d_in;
   a.n = n;
   a.kinds = kinds;
+  a.kinds8 = chunk > 0 ? nullptr : kinds8;
   a.starts = starts;
   a.ends = ends;
   a.byte_class = ctx->d_cls;
@@ -2663,12 +2796,9 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   a.doc_res = nullptr;
   const uint64_t wpb = LEX_BLOCK / WAVE;
   uint64_t blocks = (tiles + wpb - 1) / wpb;
-  blocks = std::min<uint64_t>(blocks, (uint64_t)std::max(1, ctx->cus) * 2);
+  blocks = std::min<uint64_t>(blocks, (uint64_t)std::max(1, ctx->cus) * ctx->lex_bpc);
   ctx->prof_begin(S_LEX_REDUCE, st);
-  if (ctx->comp_in_lds)
-    lex_reduce<true, false><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
-  else
-    lex_reduce<false, false><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
+  launch_lex_reduce<false>(ctx->lex_tab, (unsigned)blocks, ctx->lex_lds, a, st);
   ctx->prof_end(S_LEX_REDUCE, st);
   ctx->prof_begin(S_LEX_SCAN, st);
   lex_scan_tiles<<<(unsigned)sblocks, LEX_SCAN_BLOCK, 0, st>>>(a);
@@ -2677,11 +2807,11 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   ctx->prof_end(S_LEX_SCAN, st);
   ctx->prof_begin(S_LEX_EMIT, st);
   if (chunk > 0) {
-    lex_emit<true, false><<<(unsigned)ctiles, LEX_BLOCK, 0, st>>>(a);
+    launch_lex_emit<true, false>(ctx->dfa_lds, (unsigned)ctiles, a, st);
     lex_chunk_fixup<<<grid_for(K, 256, 1024), 256, 0, st>>>(a);
     ctx->launches++;
   } else {
-    lex_emit<false, false><<<(unsigned)ctiles, LEX_BLOCK, 0, st>>>(a);
+    launch_lex_emit<false, false>(ctx->dfa_lds, (unsigned)ctiles, a, st);
   }
   ctx->prof_end(S_LEX_EMIT, st);
   ctx->launches += 7;
@@ -2697,17 +2827,34 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   return ctx->h_small[0] ? ALPACC_ACCEPT : ALPACC_REJECT;
 }
 
-// Batched lexer: documents (offset, length) of d_in, each lexed on its own; document d's tokens go
-// to slots [tok_base_d, tok_base_d + len_d + 1) of kinds/starts/ends (tok_base = exclusive sum of
-// len + 1), its (accept, token count) to doc_res[2d], [2d+1] (device). `docs` is filled here.
-int run_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
-                  const uint64_t *lengths, uint32_t ndocs, uint32_t *kinds, uint32_t *starts,
-                  uint32_t *ends, uint32_t *doc_res, LexDoc **d_docs_out, hipStream_t st) {
+// A batched lexer pass planned on the host: every buffer sized and allocated, the document
+// descriptors staged in pinned host memory. enqueue_lex_batch then only queues one copy and the
+// kernels, with no allocation and no host synchronisation, so that a batched call can be
+// captured into a HIP graph and replayed.
+struct LexBatchPlan {
+  uint32_t ndocs;
+  uint64_t tiles, ctiles;
+  unsigned lex_blocks;
+  LexArgs a;
+  LexDoc *d_docs;  // device copy of the descriptors
+  LexDoc *h_docs;  // pinned staging of the descriptors
+  uint32_t *small;
+  uint32_t *doc_res;
+  int force_timeout, lex_tab, dfa_lds;
+  size_t lex_lds;
+};
+
+// Documents (offset, length) of d_in, each lexed on its own; document d's tokens go to slots
+// [tok_base_d, tok_base_d + len_d + 1) of kinds/starts/ends (tok_base = exclusive sum of len + 1),
+// its (accept, token count) to doc_res[2d], [2d+1] (device). h_docs: pinned, ndocs entries.
+int plan_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
+                   const uint64_t *lengths, uint32_t ndocs, uint32_t *kinds, uint32_t *starts,
+                   uint32_t *ends, uint32_t *doc_res, LexDoc *h_docs, LexBatchPlan &pl) {
+  memset(&pl, 0, sizeof(pl));
   if (!ctx->has_lexer) {
     ctx->err = "bundle has no lexer";
     return ALPACC_ERR_ARG;
   }
-  std::vector<LexDoc> docs(ndocs);
   uint64_t tiles = 0, toks = 0;
   for (uint32_t d = 0; d < ndocs; ++d) {
     if (lengths[d] >= (1ull << 31) - 2) {
@@ -2715,7 +2862,7 @@ int run_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
       return ALPACC_ERR_ARG;
     }
     const uint64_t nt = (lengths[d] + LEX_TILE - 1) / LEX_TILE;
-    docs[d] = LexDoc{offsets[d], lengths[d], toks, (uint32_t)tiles, (uint32_t)nt};
+    h_docs[d] = LexDoc{offsets[d], lengths[d], toks, (uint32_t)tiles, (uint32_t)nt};
     tiles += nt;
     toks += lengths[d] + 1;
     if (tiles >= (1ull << 31) / LEX_TILE * 2) {
@@ -2723,35 +2870,34 @@ int run_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
       return ALPACC_ERR_ARG;
     }
   }
-  LexDoc *d_docs = (LexDoc *)ctx->buf(B_DOCS, (ndocs + 1) * sizeof(LexDoc));
-  uint32_t *small = (uint32_t *)ctx->buf(B_SMALL, 64 * sizeof(uint32_t));
-  if (!d_docs || !small) {
+  pl.ndocs = ndocs;
+  pl.h_docs = h_docs;
+  pl.doc_res = doc_res;
+  pl.d_docs = (LexDoc *)ctx->buf(B_DOCS, (ndocs + 1) * sizeof(LexDoc));
+  pl.small = (uint32_t *)ctx->buf(B_SMALL, 64 * sizeof(uint32_t));
+  if (!pl.d_docs || !pl.small) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
-  CK(hipMemcpyAsync(d_docs, docs.data(), ndocs * sizeof(LexDoc), hipMemcpyHostToDevice, st));
-  *d_docs_out = d_docs;
-  clear_u32<<<grid_for(2 * (uint64_t)ndocs, 256), 256, 0, st>>>(doc_res, 2 * (uint64_t)ndocs);
-  clear_u32<<<1, 64, 0, st>>>(small, 16);
-  if (ctx->force_timeout) set_u32<<<1, 1, 0, st>>>(small + 6, 1u);
-  ctx->launches += 2;
-  if (tiles == 0) {
-    CK(hipGetLastError());
-    return 0;
-  }
+  pl.force_timeout = ctx->force_timeout ? 1 : 0;
+  pl.lex_tab = ctx->lex_tab;
+  pl.dfa_lds = ctx->dfa_lds ? 1 : 0;
+  pl.lex_lds = ctx->lex_lds;
+  pl.tiles = tiles;
+  if (tiles == 0) return 0;
   const uint64_t ctiles = (tiles + LEX_CT_WAVES - 1) / LEX_CT_WAVES;
-  uint16_t *lane_excl = (uint16_t *)ctx->buf(B_LANE_EXCL, tiles * WAVE * sizeof(uint16_t));
-  uint16_t *tile_agg = (uint16_t *)ctx->buf(B_TILE_AGG, tiles * sizeof(uint16_t));
-  uint8_t *tile_state = (uint8_t *)ctx->buf(B_TILE_STATE, tiles);
-  uint64_t *st_count = (uint64_t *)ctx->buf(B_ST_COUNT, ctiles * sizeof(uint64_t));
-  uint32_t *tile_doc = (uint32_t *)ctx->buf(B_TILE_DOC, tiles * sizeof(uint32_t));
-  if (!lane_excl || !tile_agg || !tile_state || !st_count || !tile_doc) {
+  pl.ctiles = ctiles;
+  LexArgs &a = pl.a;
+  memset(&a, 0, sizeof(a));
+  a.lane_excl = (uint16_t *)ctx->buf(B_LANE_EXCL, tiles * WAVE * sizeof(uint16_t));
+  a.tile_agg = (uint16_t *)ctx->buf(B_TILE_AGG, tiles * sizeof(uint16_t));
+  a.tile_state = (uint8_t *)ctx->buf(B_TILE_STATE, tiles);
+  a.st_count = (uint64_t *)ctx->buf(B_ST_COUNT, ctiles * sizeof(uint64_t));
+  a.tile_doc = (uint32_t *)ctx->buf(B_TILE_DOC, tiles * sizeof(uint32_t));
+  if (!a.lane_excl || !a.tile_agg || !a.tile_state || !a.st_count || !a.tile_doc) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
-  clear_u32<<<grid_for(2 * ctiles, 256), 256, 0, st>>>((uint32_t *)st_count, 2 * ctiles);
-  LexArgs a;
-  memset(&a, 0, sizeof(a));
   a.in = d_in;
   a.n = 0;
   a.kinds = kinds;
@@ -2770,33 +2916,55 @@ int run_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
   a.ignore = ctx->ignore;
   a.num_tiles = (uint32_t)tiles;
   a.num_ctiles = (uint32_t)ctiles;
-  a.lane_excl = lane_excl;
-  a.tile_agg = tile_agg;
-  a.tile_state = tile_state;
-  a.tile_counter = small;
-  a.st_count = st_count;
-  a.result = small + 4;
-  a.docs = d_docs;
-  a.tile_doc = tile_doc;
+  a.tile_counter = pl.small;
+  a.result = pl.small + 4;
+  a.docs = pl.d_docs;
   a.ndocs = ndocs;
   a.doc_res = doc_res;
-  lex_tile_docs<<<ndocs, 256, 0, st>>>(a);
   const uint64_t wpb = LEX_BLOCK / WAVE;
-  uint64_t blocks = (tiles + wpb - 1) / wpb;
-  blocks = std::min<uint64_t>(blocks, (uint64_t)std::max(1, ctx->cus) * 2);
+  const uint64_t blocks = (tiles + wpb - 1) / wpb;
+  pl.lex_blocks = (unsigned)std::min<uint64_t>(blocks, (uint64_t)std::max(1, ctx->cus) * ctx->lex_bpc);
+  return 0;
+}
+
+void enqueue_lex_batch(alpacc_ctx *ctx, const LexBatchPlan &pl, hipStream_t st) {
+  (void)hipMemcpyAsync(pl.d_docs, pl.h_docs, pl.ndocs * sizeof(LexDoc), hipMemcpyHostToDevice, st);
+  clear_u32<<<grid_for(2 * (uint64_t)pl.ndocs, 256), 256, 0, st>>>(pl.doc_res, 2 * (uint64_t)pl.ndocs);
+  clear_u32<<<1, 64, 0, st>>>(pl.small, 16);
+  if (pl.force_timeout) set_u32<<<1, 1, 0, st>>>(pl.small + 6, 1u);
+  ctx->launches += 2;
+  if (pl.tiles == 0) return;
+  const LexArgs &a = pl.a;
+  clear_u32<<<grid_for(2 * pl.ctiles, 256), 256, 0, st>>>((uint32_t *)a.st_count, 2 * pl.ctiles);
+  lex_tile_docs<<<pl.ndocs, 256, 0, st>>>(a);
   ctx->prof_begin(S_LEX_REDUCE, st);
-  if (ctx->comp_in_lds)
-    lex_reduce<true, true><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
-  else
-    lex_reduce<false, true><<<(unsigned)blocks, LEX_BLOCK, ctx->lex_lds, st>>>(a);
+  launch_lex_reduce<true>(pl.lex_tab, pl.lex_blocks, pl.lex_lds, a, st);
   ctx->prof_end(S_LEX_REDUCE, st);
   ctx->prof_begin(S_LEX_SCAN, st);
-  lex_scan_docs<<<ndocs, LEX_SCAN_BLOCK, 0, st>>>(a);
+  lex_scan_docs<<<pl.ndocs, LEX_SCAN_BLOCK, 0, st>>>(a);
   ctx->prof_end(S_LEX_SCAN, st);
   ctx->prof_begin(S_LEX_EMIT, st);
-  lex_emit<false, true><<<(unsigned)ctiles, LEX_BLOCK, 0, st>>>(a);
+  launch_lex_emit<false, true>(pl.dfa_lds != 0, (unsigned)pl.ctiles, a, st);
   ctx->prof_end(S_LEX_EMIT, st);
   ctx->launches += 5;
+}
+
+// The batched lexer queued on `st` (descriptors through the context's pinned staging);
+// *d_docs_out receives the device descriptors.
+int run_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
+                  const uint64_t *lengths, uint32_t ndocs, uint32_t *kinds, uint32_t *starts,
+                  uint32_t *ends, uint32_t *doc_res, LexDoc **d_docs_out, hipStream_t st) {
+  LexDoc *h_docs = (LexDoc *)ctx->stage((ndocs + 1) * sizeof(LexDoc), st);
+  if (!h_docs) {
+    ctx->err = "pinned staging allocation failed";
+    return ALPACC_ERR_HIP;
+  }
+  LexBatchPlan pl;
+  int rc = plan_lex_batch(ctx, d_in, offsets, lengths, ndocs, kinds, starts, ends, doc_res, h_docs,
+                          pl);
+  if (rc) return rc;
+  enqueue_lex_batch(ctx, pl, st);
+  *d_docs_out = pl.d_docs;
   CK(hipGetLastError());
   return 0;
 }
@@ -2813,6 +2981,10 @@ void parse_args_init(alpacc_ctx *ctx, PArgs &a) {
   a.index8 = ctx->d_index8;
   a.index32 = ctx->d_pair_index;
   a.index_len = (uint32_t)ctx->index_len;
+  a.hkeys = ctx->d_hkeys;
+  a.hvals = ctx->d_hvals;
+  a.hbits = ctx->hbits;
+  a.hprobe = ctx->hprobe;
   a.einfo = ctx->d_einfo;
   a.E = ctx->E;
   a.stackw = ctx->d_stackw;
@@ -2824,11 +2996,12 @@ void parse_args_init(alpacc_ctx *ctx, PArgs &a) {
 
 // keys, tile totals and the tile scan (ntiles_max: tiles to cover, the device count in batch)
 template <bool BATCH>
-int parse_phase1(alpacc_ctx *ctx, PArgs &a, uint64_t ntiles_max, hipStream_t st) {
+int parse_phase1(alpacc_ctx *ctx, PArgs &a, uint64_t ntiles_max, hipStream_t st,
+                 SegTup *btot = nullptr) {
   ctx->prof_begin(S_PARSE_KEYS, st);
   const uint64_t nsuper = (ntiles_max + PKK_TILES - 1) / PKK_TILES;
   const unsigned kgrid =
-      (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(nsuper, (uint64_t)std::max(1, ctx->cus) * 2));
+      (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(nsuper, (uint64_t)std::max(1, ctx->cus) * ctx->keys_bpc));
   const bool small_w = ctx->q + ctx->k <= 4;
   if (ctx->d_index8 && small_w)
     parse_keys2<true, BATCH, 4><<<kgrid, PKK_THREADS, ctx->index_len, st>>>(a);
@@ -2839,7 +3012,7 @@ int parse_phase1(alpacc_ctx *ctx, PArgs &a, uint64_t ntiles_max, hipStream_t st)
   else
     parse_keys2<false, BATCH, PK_MAXW><<<kgrid, PKK_THREADS, 0, st>>>(a);
   const uint64_t nb = std::max<uint64_t>(1, (ntiles_max + PSB - 1) / PSB);
-  SegTup *btot = (SegTup *)ctx->buf(B_BTOT, (nb + 1) * sizeof(SegTup));
+  if (!btot) btot = (SegTup *)ctx->buf(B_BTOT, (nb + 1) * sizeof(SegTup));
   if (!btot) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
@@ -2891,13 +3064,13 @@ int run_search(alpacc_ctx *ctx, const PArgs &a, const ElemBufs &e, uint64_t B, u
   if (rc) return rc;
   if (B > 0) {
     ctx->prof_begin(S_BRACKET_MATCH, st);
-    bracket_match<BATCH><<<grid_for(B, 256), 256, 0, st>>>(e.br, B, lvb, flags, a);
+    launch_bracket_match<BATCH>(ctx->search_bs, ctx->search_cap, e.br, B, lvb, flags, a, st);
     ctx->prof_end(S_BRACKET_MATCH, st);
     ctx->launches++;
   }
   if (N > 0) {
     ctx->prof_begin(S_PARSE_PARENTS, st);
-    parse_parents<BATCH><<<grid_for(N, 256), 256, 0, st>>>(N, lvs, parent, a);
+    launch_parse_parents<BATCH>(ctx->search_bs, ctx->search_cap, N, lvs, parent, a, st);
     ctx->prof_end(S_PARSE_PARENTS, st);
     ctx->launches++;
   }
@@ -2905,14 +3078,18 @@ int run_search(alpacc_ctx *ctx, const PArgs &a, const ElemBufs &e, uint64_t B, u
   return 0;
 }
 
-// lexed = true: the tokens come straight from run_lex(sync = false) in this call; T is then an
-// upper bound (the input length), the key and scan kernels read the token count from the
-// lexer's device words, and the one host synchronisation below also reads the lexer's verdict.
+// The parser on a token stream of T tokens, with one host synchronisation after the tile scan,
+// which sizes the element scratch (so that parse_expand2 runs one workgroup per actual tile) and
+// decides the missing-pair and depth checks.
+// lexed = true: the tokens come straight from run_lex(sync = false) in this call (kinds as bytes
+// in kinds8); T is then an upper bound (the input length), the key and scan kernels read the
+// token count from the lexer's device words, and the synchronisation also reads the lexer's
+// verdict.
 template <typename S>
 int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, const S *ends,
                      uint64_t T, uint32_t *parent, uint32_t *node, S *span, uint64_t cap,
                      uint64_t *num_nodes, hipStream_t st, bool productions_only = false,
-                     bool lexed = false) {
+                     bool lexed = false, const uint8_t *kinds8 = nullptr) {
   if (!ctx->has_parser) {
     ctx->err = "bundle has no parser";
     return ALPACC_ERR_NO_PARSER;
@@ -2938,6 +3115,7 @@ int run_parse_tokens(alpacc_ctx *ctx, const uint32_t *kinds, const S *starts, co
   PArgs a;
   parse_args_init(ctx, a);
   a.kinds = kinds;
+  a.kinds8 = kinds8;
   a.T = T;
   a.P = P;
   a.T_dev = lexed ? small + 5 : nullptr;  // the lexer's result words: verdict, count, time-out
@@ -3044,17 +3222,32 @@ __global__ void lb_finish(const LexDoc *docs, const uint32_t *doc_res, uint32_t 
   out[d] = r;
 }
 
-// Batched lex + parse of many documents: one lexer pass and one parser pass over all of them,
-// with one host synchronisation (to size the bracket / production scratch).
-int run_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
-                    const uint64_t *lengths, uint32_t ndocs, uint32_t *parent, uint32_t *node,
-                    uint32_t *span, uint64_t cap, uint64_t *total_nodes,
-                    alpacc_doc_result *results, hipStream_t st) {
-  if (!ctx->has_parser) {
-    ctx->err = "bundle has no parser";
-    return ALPACC_ERR_NO_PARSER;
-  }
-  *total_nodes = 0;
+// A batched lex + parse planned on the host (as LexBatchPlan): the bracket and production scratch
+// is sized before the launch, for the productions from the output capacity and for the brackets
+// from an estimate, so that the whole call is one queue with no host synchronisation before its
+// end. A batch whose totals exceed those capacities writes no elements (batch_elems_fit); the
+// host then plans it again with the sizes the first pass reported.
+struct ParseBatchPlan {
+  LexBatchPlan lx;
+  PArgs a;
+  uint64_t tiles_max;
+  SegTup *btot;
+  ElemBufs e;
+  uint64_t nbc, npc;  // 64-element chunks of the bracket / production capacities
+  Levels lvb, lvs;
+  uint32_t *parent, *node, *span;
+  const uint32_t *starts, *ends;
+  alpacc_doc_result *d_results, *h_results;
+  uint32_t *h_words;  // pinned copy of the result words small[0 .. 32)
+  size_t xlds;
+  unsigned search_bs, search_cap;
+};
+
+int plan_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
+                     const uint64_t *lengths, uint32_t ndocs, uint32_t *parent, uint32_t *node,
+                     uint32_t *span, uint64_t cap, uint64_t bcap, ParseBatchPlan &pl,
+                     hipStream_t st) {
+  memset(&pl, 0, sizeof(pl));
   uint64_t nbytes = 0, tiles_max = 0;
   for (uint32_t d = 0; d < ndocs; ++d) {
     nbytes += lengths[d];
@@ -3073,7 +3266,9 @@ int run_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offset
     ctx->err = "batch too large for 32-bit element counts";
     return ALPACC_ERR_ARG;
   }
-  uint32_t *kinds = (uint32_t *)ctx->buf(B_TOK_KIND, tslots * 4);
+  const uint64_t ncap = std::min<uint64_t>(cap, 1ull << 31);
+  const uint64_t nbc = (bcap + WAVE - 1) / WAVE, npc = (ncap + WAVE - 1) / WAVE;
+  uint8_t *kinds = (uint8_t *)ctx->buf(B_TOK_KIND, tslots + 16);  // kinds as bytes
   uint32_t *starts = (uint32_t *)ctx->buf(B_TOK_START, tslots * 4);
   uint32_t *ends = (uint32_t *)ctx->buf(B_TOK_END, tslots * 4);
   uint32_t *doc_res = (uint32_t *)ctx->buf(B_DOC_RES, (2ull * ndocs + 2) * 4);
@@ -3083,95 +3278,257 @@ int run_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offset
   uint16_t *ent = (uint16_t *)ctx->buf(B_ENT, (tiles_max * PK_TILE + 8) * sizeof(uint16_t));
   PTup *tiles = (PTup *)ctx->buf(B_PTILES, (tiles_max + 1) * sizeof(PTup));
   PTup *pre = (PTup *)ctx->buf(B_PPRE, (tiles_max + 1) * sizeof(PTup));
-  alpacc_doc_result *d_results =
-      (alpacc_doc_result *)ctx->buf(B_RESULTS, (ndocs + 1) * sizeof(alpacc_doc_result));
+  pl.d_results = (alpacc_doc_result *)ctx->buf(B_RESULTS, (ndocs + 1) * sizeof(alpacc_doc_result));
+  const uint64_t nbs = std::max<uint64_t>(1, (tiles_max + PSB - 1) / PSB);
+  pl.btot = (SegTup *)ctx->buf(B_BTOT, (nbs + 1) * sizeof(SegTup));
+  pl.e.depth = (int32_t *)ctx->buf(B_DEPTH, (bcap + 16) * sizeof(int32_t));
+  pl.e.br = (uint16_t *)ctx->buf(B_BR, (bcap + 16) * sizeof(uint16_t));
+  pl.e.dl1 = (int32_t *)ctx->buf(B_DL1, (nbc + 16) * sizeof(int32_t));
+  pl.e.sx = (int32_t *)ctx->buf(B_SX, (ncap + 16) * sizeof(int32_t));
+  pl.e.sl1 = (int32_t *)ctx->buf(B_SL1, (npc + 16) * sizeof(int32_t));
+  uint32_t *chunk_doc = (uint32_t *)ctx->buf(B_CHUNK_DOC, (npc + 2) * 4);
+  const size_t docs_bytes = ((ndocs + 1) * sizeof(LexDoc) + 255) & ~(size_t)255;
+  const size_t res_bytes = ((ndocs + 1) * sizeof(alpacc_doc_result) + 255) & ~(size_t)255;
+  uint8_t *stg = ctx->stage(docs_bytes + res_bytes + 256, st);
   if (!kinds || !starts || !ends || !doc_res || !pdocs || !doc_flags || !ptile_doc || !ent ||
-      !tiles || !pre || !d_results) {
+      !tiles || !pre || !pl.d_results || !pl.btot || !pl.e.depth || !pl.e.br || !pl.e.dl1 ||
+      !pl.e.sx || !pl.e.sl1 || !chunk_doc || !stg) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
-  LexDoc *ldocs = nullptr;
-  int rc = run_lex_batch(ctx, d_in, offsets, lengths, ndocs, kinds, starts, ends, doc_res, &ldocs,
-                         st);
+  pl.h_results = reinterpret_cast<alpacc_doc_result *>(stg + docs_bytes);
+  pl.h_words = reinterpret_cast<uint32_t *>(stg + docs_bytes + res_bytes);
+  int rc = plan_lex_batch(ctx, d_in, offsets, lengths, ndocs, nullptr, starts, ends, doc_res,
+                          reinterpret_cast<LexDoc *>(stg), pl.lx);
   if (rc) return rc;
-  uint32_t *small = (uint32_t *)ctx->buf(B_SMALL, 64 * sizeof(uint32_t));
-  uint64_t *ntiles_dev = reinterpret_cast<uint64_t *>(small + 24);
-  PTup *total = reinterpret_cast<PTup *>(small + 16);
-  PArgs a;
+  pl.lx.a.kinds8 = kinds;
+  uint32_t *small = pl.lx.small;
+  PArgs &a = pl.a;
   parse_args_init(ctx, a);
-  a.kinds = kinds;
+  a.kinds = nullptr;
+  a.kinds8 = kinds;
   a.ent = ent;
   a.tiles = tiles;
   a.pre = pre;
-  a.total = total;
+  a.total = reinterpret_cast<PTup *>(small + 16);
   a.flags = small + 8;
   a.pdocs = pdocs;
   a.ptile_doc = ptile_doc;
-  a.ntiles_dev = ntiles_dev;
+  a.ntiles_dev = reinterpret_cast<uint64_t *>(small + 24);
   a.ndocs = ndocs;
   a.doc_flags = doc_flags;
+  a.chunk_doc = chunk_doc;
+  a.bcap = bcap;
+  a.ncap = ncap;
+  pl.tiles_max = tiles_max;
+  pl.nbc = nbc;
+  pl.npc = npc;
+  pl.parent = parent;
+  pl.node = node;
+  pl.span = span;
+  pl.starts = starts;
+  pl.ends = ends;
+  pl.lvb.nlev = pl.lvs.nlev = 0;
+  if (bcap > 0) rc = plan_levels(ctx, pl.e.depth, pl.e.dl1, bcap, pl.lvb, B_LEVELS);
+  if (!rc && ncap > 0) rc = plan_levels(ctx, pl.e.sx, pl.e.sl1, ncap, pl.lvs, B_LEVELS_S);
+  if (rc) return rc;
+  pl.xlds = px_lds_bytes(ctx->nstackw, ctx->nprodw, ctx->E);
+  pl.search_bs = ctx->search_bs;
+  pl.search_cap = ctx->search_cap;
+  return 0;
+}
+
+// Queues the planned call on `st`: copies and kernels only (no allocation, no synchronisation).
+void enqueue_parse_batch(alpacc_ctx *ctx, const ParseBatchPlan &pl, hipStream_t st) {
+  enqueue_lex_batch(ctx, pl.lx, st);
+  PArgs a = pl.a;
+  const uint32_t ndocs = pl.lx.ndocs;
   ctx->prof_begin(S_PARSE, st);
-  pb_setup<<<1, PS_THREADS, 0, st>>>(a, ldocs, doc_res, ntiles_dev);
+  pb_setup<<<1, PS_THREADS, 0, st>>>(a, pl.lx.d_docs, pl.lx.doc_res,
+                                      const_cast<uint64_t *>(a.ntiles_dev));
   pb_tile_docs<<<ndocs, 256, 0, st>>>(a);
   ctx->launches += 2;
-  rc = parse_phase1<true>(ctx, a, tiles_max, st);
-  if (rc) return rc;
+  (void)parse_phase1<true>(ctx, a, pl.tiles_max, st, pl.btot);
   pb_docs<<<(ndocs + 255) / 256, 256, 0, st>>>(a);
-  ctx->launches++;
-  CK(hipGetLastError());
-  // words 4..7: the lexer's results (time-out flag at 6); 8..11: parser flags (overflow at 9);
-  // 16..21: grand total; 24..25: position tiles
-  CK(hipMemcpyAsync(ctx->h_small, small, 28 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  CK(hipStreamSynchronize(st));
-  if (ctx->h_small[6]) {
-    ctx->err = "lexer look-back did not complete";
-    return ALPACC_ERR_TIMEOUT;
-  }
-  PTup tot;
-  memcpy(&tot, ctx->h_small + 16, sizeof(PTup));
-  uint64_t ntiles;
-  memcpy(&ntiles, ctx->h_small + 24, sizeof(uint64_t));
-  if (ctx->h_small[9] || tot.nb < 0 || tot.np < 0) {
-    ctx->err = "batch too large: more than 2^31 brackets or productions";
-    return ALPACC_ERR_ARG;
-  }
-  const uint64_t B = (uint32_t)tot.nb, N = (uint32_t)tot.np;
-  *total_nodes = N;
-  ctx->last_counts[0] = 0;
-  ctx->last_counts[1] = ntiles * PK_TILE;
-  ctx->last_counts[2] = B;
-  ctx->last_counts[3] = N;
-  const int write_nodes = N <= cap ? 1 : 0;
-  ElemBufs e;
-  rc = elem_bufs(ctx, B, N, e, st);
-  if (rc) return rc;
-  a.chunk_doc = (uint32_t *)ctx->buf(B_CHUNK_DOC, ((N + WAVE - 1) / WAVE + 1) * 4);
-  if (!a.chunk_doc) {
-    ctx->err = "scratch allocation failed";
-    return ALPACC_ERR_HIP;
-  }
-  if (ntiles > 0) {
-    const size_t xlds = px_lds_bytes(ctx->nstackw, ctx->nprodw, ctx->E);
+  elem_init<<<grid_for(std::max(pl.nbc, pl.npc) + 16, 256, 4096), 256, 0, st>>>(
+      pl.e.dl1, pl.nbc + 16, pl.e.sl1, pl.npc + 16);
+  ctx->launches += 2;
+  if (pl.tiles_max > 0) {
     ctx->prof_begin(S_PARSE_EXPAND, st);
-    parse_expand2<uint32_t, true><<<(unsigned)ntiles, PK_THREADS, xlds, st>>>(
-        a, starts, ends, e.depth, e.br, e.dl1, e.sx, e.sl1, node, span, write_nodes);
+    parse_expand2<uint32_t, true><<<(unsigned)pl.tiles_max, PK_THREADS, pl.xlds, st>>>(
+        a, pl.starts, pl.ends, pl.e.depth, pl.e.br, pl.e.dl1, pl.e.sx, pl.e.sl1, pl.node, pl.span,
+        1);
     ctx->prof_end(S_PARSE_EXPAND, st);
     ctx->launches++;
   }
-  rc = run_search<true>(ctx, a, e, B, write_nodes ? N : 0, parent, a.flags, st);
-  if (rc) return rc;
-  pb_finish<<<(ndocs + 255) / 256, 256, 0, st>>>(a, d_results);
-  ctx->launches++;
-  CK(hipGetLastError());
-  ctx->prof_end(S_PARSE, st);
-  CK(hipMemcpyAsync(results, d_results, ndocs * sizeof(alpacc_doc_result), hipMemcpyDeviceToHost,
-                    st));
-  CK(hipStreamSynchronize(st));
-  if (!write_nodes) {
-    ctx->err = "output capacity too small";
-    return ALPACC_ERR_CAPACITY;
+  (void)fill_levels(ctx, pl.lvb, pl.lvs, st);
+  if (a.bcap > 0) {
+    ctx->prof_begin(S_BRACKET_MATCH, st);
+    launch_bracket_match<true>(pl.search_bs, pl.search_cap, pl.e.br, a.bcap, pl.lvb, a.flags, a,
+                               st);
+    ctx->prof_end(S_BRACKET_MATCH, st);
+    ctx->launches++;
   }
-  return ALPACC_ACCEPT;
+  if (a.ncap > 0) {
+    ctx->prof_begin(S_PARSE_PARENTS, st);
+    launch_parse_parents<true>(pl.search_bs, pl.search_cap, a.ncap, pl.lvs, pl.parent, a, st);
+    ctx->prof_end(S_PARSE_PARENTS, st);
+    ctx->launches++;
+  }
+  pb_finish<<<(ndocs + 255) / 256, 256, 0, st>>>(a, pl.d_results);
+  ctx->launches++;
+  ctx->prof_end(S_PARSE, st);
+  (void)hipMemcpyAsync(pl.h_results, pl.d_results, ndocs * sizeof(alpacc_doc_result),
+                       hipMemcpyDeviceToHost, st);
+  (void)hipMemcpyAsync(pl.h_words, pl.lx.small, 32 * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+}
+
+// Runs a planned call on `st`: queued directly, or through a HIP graph. graph_mode 1 captures
+// the queue when a call's plan (every size, pointer and flag) equals the previous call's, and
+// replays that graph while the plans keep repeating; graph_mode 2 captures every new plan.
+// Profiling (HIP events per stage) always queues directly.
+int launch_parse_batch(alpacc_ctx *ctx, const ParseBatchPlan &pl, hipStream_t st) {
+  if (ctx->graph_mode == 0 || ctx->prof) {
+    enqueue_parse_batch(ctx, pl, st);
+    CK(hipGetLastError());
+    return 0;
+  }
+  const uint8_t *pb = reinterpret_cast<const uint8_t *>(&pl);
+  std::vector<uint8_t> plan(pb, pb + sizeof(pl));
+  if (ctx->gexec && plan == ctx->gplan) {
+    CK(hipGraphLaunch(ctx->gexec, st));
+    ctx->graph_launches++;
+    return 0;
+  }
+  if (ctx->graph_mode == 1 && plan != ctx->last_plan) {
+    ctx->last_plan.swap(plan);
+    enqueue_parse_batch(ctx, pl, st);
+    CK(hipGetLastError());
+    return 0;
+  }
+  ctx->drop_graph();
+  if (!ctx->cap_stream) CK(hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking));
+  CK(hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
+  enqueue_parse_batch(ctx, pl, ctx->cap_stream);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->cap_stream, &graph);
+  const hipError_t el = hipGetLastError();
+  if (ec != hipSuccess || el != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    ctx->err = std::string("graph capture failed: ") + hipGetErrorString(ec != hipSuccess ? ec : el);
+    return ALPACC_ERR_HIP;
+  }
+  const hipError_t ei = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    ctx->gexec = nullptr;
+    ctx->err = std::string("graph instantiation failed: ") + hipGetErrorString(ei);
+    return ALPACC_ERR_HIP;
+  }
+  ctx->gplan.assign(pb, pb + sizeof(pl));
+  ctx->last_plan = ctx->gplan;
+  ctx->graph_captures++;
+  CK(hipGraphLaunch(ctx->gexec, st));
+  ctx->graph_launches++;
+  return 0;
+}
+
+// Batched lex + parse of many documents: one lexer pass and one parser pass over all of them,
+// queued as a whole (or replayed as a graph) and followed by one host synchronisation. A batch
+// with more brackets than the scratch was sized for runs again with the exact size.
+int run_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets,
+                    const uint64_t *lengths, uint32_t ndocs, uint32_t *parent, uint32_t *node,
+                    uint32_t *span, uint64_t cap, uint64_t *total_nodes,
+                    alpacc_doc_result *results, hipStream_t st) {
+  if (!ctx->has_parser) {
+    ctx->err = "bundle has no parser";
+    return ALPACC_ERR_NO_PARSER;
+  }
+  *total_nodes = 0;
+  uint64_t nbytes = 0;
+  for (uint32_t d = 0; d < ndocs; ++d) nbytes += lengths[d];
+  // brackets: at most max_nb per key position (T + 2 <= length + 2 per document)
+  const uint64_t b_bound = (nbytes + 2ull * ndocs) * std::max<uint32_t>(ctx->max_nb, 1);
+  uint64_t bcap = std::min<uint64_t>(
+      b_bound, std::max<uint64_t>(4096, (uint64_t)(ctx->b_per_byte * (double)nbytes)));
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    ParseBatchPlan pl;
+    int rc = plan_parse_batch(ctx, d_in, offsets, lengths, ndocs, parent, node, span, cap, bcap,
+                              pl, st);
+    if (rc) return rc;
+    rc = launch_parse_batch(ctx, pl, st);
+    if (rc) return rc;
+    CK(hipStreamSynchronize(st));
+    const uint32_t *h = pl.h_words;
+    // words 4..7: the lexer's results (time-out flag at 6); 8..11: parser flags (overflow at 9);
+    // 16..21: grand total; 24..25: position tiles
+    if (h[6]) {
+      ctx->err = "lexer look-back did not complete";
+      return ALPACC_ERR_TIMEOUT;
+    }
+    PTup tot;
+    memcpy(&tot, h + 16, sizeof(PTup));
+    uint64_t ntiles;
+    memcpy(&ntiles, h + 24, sizeof(uint64_t));
+    if (h[9] || tot.nb < 0 || tot.np < 0) {
+      ctx->err = "batch too large: more than 2^31 brackets or productions";
+      return ALPACC_ERR_ARG;
+    }
+    const uint64_t B = (uint32_t)tot.nb, N = (uint32_t)tot.np;
+    *total_nodes = N;
+    ctx->last_counts[0] = 0;
+    ctx->last_counts[1] = ntiles * PK_TILE;
+    ctx->last_counts[2] = B;
+    ctx->last_counts[3] = N;
+    if (N > cap) {
+      ctx->err = "output capacity too small";
+      return ALPACC_ERR_CAPACITY;
+    }
+    if (B > bcap) {  // the first pass wrote nothing: size the brackets for this batch
+      bcap = B + B / 8 + 4096;
+      if (nbytes) ctx->b_per_byte = std::max(ctx->b_per_byte, (double)bcap / (double)nbytes);
+      continue;
+    }
+    memcpy(results, pl.h_results, ndocs * sizeof(alpacc_doc_result));
+    return ALPACC_ACCEPT;
+  }
+  ctx->err = "bracket scratch sizing did not converge";
+  return ALPACC_ERR_HIP;
+}
+
+// Where the lexer's tables sit: lex_reduce keeps the (endo x class) table in LDS when it fits in
+// 160 KiB (and the composition table too when both fit in 48 KiB), else reads both from L2;
+// lex_emit keeps the DFA table in LDS up to LEX_MAX_DFA entries, else reads it from L2. With the
+// tuning choice lexer_tables_in_lds = 0 every table stays in L2.
+int place_lexer_tables(alpacc_ctx *ctx, std::string &err) {
+  size_t lds = (size_t)ctx->e * ctx->C * 2 + 256;
+  lds = (lds + 15) & ~(size_t)15;
+  const size_t comp_bytes = (size_t)ctx->e * ctx->e * 2;
+  if (!ctx->lex_tables_lds) {
+    ctx->lex_tab = 0;
+    lds = 256;
+  } else if (lds + comp_bytes <= 48 * 1024) {
+    ctx->lex_tab = 2;
+    lds += comp_bytes;
+  } else if (lds <= 160 * 1024) {
+    ctx->lex_tab = 1;
+  } else {
+    ctx->lex_tab = 0;
+    lds = 256;
+  }
+  ctx->dfa_lds = ctx->lex_tables_lds && (size_t)ctx->m1 * ctx->C <= (size_t)LEX_MAX_DFA;
+  if (lds > 64 * 1024) {
+    const int want = (int)lds;
+    if (hipFuncSetAttribute((const void *)lex_reduce<1, false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess ||
+        hipFuncSetAttribute((const void *)lex_reduce<1, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess) {
+      err = "lexer tables do not fit in LDS";
+      return -1;
+    }
+  }
+  ctx->lex_lds = lds;
+  return 0;
 }
 
 // Per pair entry totals and per element relative values (see the parser kernels) from the
@@ -3242,17 +3599,59 @@ bool build_parser_tables(alpacc_ctx *ctx, const alpacc_tables *t, std::string &e
       return false;
     }
   }
-  for (uint64_t i = 0; i < t->pair_index_len; ++i) {
-    if (t->pair_index[i] != 0xFFFFFFFFu && t->pair_index[i] >= E) {
-      err = "pair index entry out of range";
+  // number of key windows, saturating at 2^64 - 1
+  uint64_t want_len = 1;
+  for (uint32_t j = 0; j < t->q + t->k; ++j)
+    want_len = want_len > ~0ull / std::max<uint64_t>(t->num_terminals, 1)
+                   ? ~0ull : want_len * t->num_terminals;
+  const bool hashed = t->pair_hash_len > 0;
+  if (hashed) {
+    const uint64_t H = t->pair_hash_len;
+    if ((H & (H - 1)) != 0 || H < 2 || H > (1ull << 32) || !t->pair_hash_keys ||
+        !t->pair_hash_vals || want_len == ~0ull) {
+      err = "hashed pair table malformed";
       return false;
     }
-  }
-  uint64_t want_len = 1;
-  for (uint32_t j = 0; j < t->q + t->k; ++j) want_len *= t->num_terminals;
-  if (want_len != t->pair_index_len) {
-    err = "pair index length is not num_terminals^(q+k)";
-    return false;
+    uint32_t bits = 0;
+    while ((1ull << bits) < H) ++bits;
+    // every stored key must be reachable from its home slot; the longest probe bounds the search
+    uint64_t probe_max = 1, used = 0;
+    for (uint64_t i = 0; i < H; ++i) {
+      const uint64_t key = t->pair_hash_keys[i];
+      if (key == ~0ull) continue;
+      ++used;
+      if (key >= want_len || t->pair_hash_vals[i] >= E) {
+        err = "hashed pair table entry out of range";
+        return false;
+      }
+      uint64_t h = (key * 0x9E3779B97F4A7C15ull) >> (64 - bits), d = 1;
+      while (h != i) {
+        if (t->pair_hash_keys[h] == ~0ull) {
+          err = "hashed pair table: a key is not reachable from its home slot";
+          return false;
+        }
+        h = (h + 1) & (H - 1);
+        ++d;
+      }
+      probe_max = std::max(probe_max, d);
+    }
+    if (used == H) {
+      err = "hashed pair table has no free slot";
+      return false;
+    }
+    ctx->hbits = bits;
+    ctx->hprobe = (uint32_t)std::min<uint64_t>(probe_max + 1, H);
+  } else {
+    for (uint64_t i = 0; i < t->pair_index_len; ++i) {
+      if (t->pair_index[i] != 0xFFFFFFFFu && t->pair_index[i] >= E) {
+        err = "pair index entry out of range";
+        return false;
+      }
+    }
+    if (want_len != t->pair_index_len || !t->pair_index) {
+      err = "pair index length is not num_terminals^(q+k)";
+      return false;
+    }
   }
   if (stackw.empty()) stackw.push_back(0);
   if (prodw.empty()) prodw.push_back(make_uint2(0, 0));
@@ -3261,8 +3660,14 @@ bool build_parser_tables(alpacc_ctx *ctx, const alpacc_tables *t, std::string &e
     return false;
   }
   ctx->index_len = t->pair_index_len;
-  ctx->d_pair_index = upload(t->pair_index, t->pair_index_len, err);
-  if (E < 255 && t->pair_index_len <= (uint64_t)PK_IDX8_MAX) {
+  if (hashed) {
+    ctx->d_hkeys = upload(t->pair_hash_keys, t->pair_hash_len, err);
+    ctx->d_hvals = upload(t->pair_hash_vals, t->pair_hash_len, err);
+    ctx->table_bytes += t->pair_hash_len * 12;
+  } else {
+    ctx->d_pair_index = upload(t->pair_index, t->pair_index_len, err);
+  }
+  if (!hashed && E < 255 && t->pair_index_len <= (uint64_t)PK_IDX8_MAX) {
     std::vector<uint8_t> idx8(t->pair_index_len);
     for (uint64_t i = 0; i < t->pair_index_len; ++i)
       idx8[i] = t->pair_index[i] == 0xFFFFFFFFu ? (uint8_t)0xFF : (uint8_t)t->pair_index[i];
@@ -3274,6 +3679,9 @@ bool build_parser_tables(alpacc_ctx *ctx, const alpacc_tables *t, std::string &e
   ctx->d_stackw = upload(stackw.data(), stackw.size(), err);
   ctx->d_prodw = upload(prodw.data(), prodw.size(), err);
   ctx->d_ecan = upload(ecan.data(), ecan.size(), err);
+  ctx->table_bytes += ctx->index_len * 4 + (ctx->d_index8 ? ctx->index_len : 0) +
+                      info.size() * sizeof(EntInfo) + stackw.size() * 4 + prodw.size() * 8 +
+                      ecan.size() * 4;
   return err.empty();
 }
 
@@ -3328,35 +3736,15 @@ alpacc_ctx *alpacc_create(const alpacc_tables *t, int device) {
       alpacc_destroy(ctx);
       return nullptr;
     }
-    if ((size_t)ctx->m1 * ctx->C > (size_t)LEX_MAX_DFA) {
-      err = "lexer DFA table too large";
+    if (place_lexer_tables(ctx, err) != 0) {
       alpacc_destroy(ctx);
       return nullptr;
     }
-    size_t lds = (size_t)ctx->e * ctx->C * 2 + 256;
-    lds = (lds + 15) & ~(size_t)15;
-    const size_t comp_bytes = (size_t)ctx->e * ctx->e * 2;
-    ctx->comp_in_lds = lds + comp_bytes <= 48 * 1024;
-    if (ctx->comp_in_lds) lds += comp_bytes;
-    if (lds > 160 * 1024) {
-      err = "lexer tables do not fit in LDS";
-      alpacc_destroy(ctx);
-      return nullptr;
-    }
-    if (lds > 64 * 1024) {
-      const int want = (int)lds;
-      if (hipFuncSetAttribute((const void *)lex_reduce<false, false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess ||
-          hipFuncSetAttribute((const void *)lex_reduce<false, true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess) {
-        err = "lexer tables do not fit in LDS";
-        alpacc_destroy(ctx);
-        return nullptr;
-      }
-    }
-    ctx->lex_lds = lds;
   }
-  if (t->num_productions > 0 && t->pair_index) {
+  if (ctx->has_lexer)
+    ctx->table_bytes += 256 + (uint64_t)ctx->e * ctx->C * 2 + (uint64_t)ctx->e * ctx->e * 2 +
+                        (uint64_t)ctx->e * ctx->m1 + (uint64_t)ctx->m1 * ctx->C * 2 + ctx->m1;
+  if (t->num_productions > 0 && (t->pair_index || t->pair_hash_len)) {
     ctx->has_parser = true;
     ctx->q = t->q;
     ctx->k = t->k;
@@ -3380,10 +3768,14 @@ void alpacc_destroy(alpacc_ctx *ctx) {
     if (s.first) (void)hipFree(s.first);
   void *tabs[] = {ctx->d_cls,    ctx->d_step,     ctx->d_comp,   ctx->d_apply,
                   ctx->d_dfa,    ctx->d_accept,   ctx->d_pair_index, ctx->d_index8,
+                  ctx->d_hkeys,  ctx->d_hvals,
                   ctx->d_einfo,  ctx->d_stackw,   ctx->d_prodw,  ctx->d_ecan};
   for (void *p : tabs)
     if (p) (void)hipFree(p);
   if (ctx->h_small) (void)hipHostFree(ctx->h_small);
+  ctx->drop_graph();
+  if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
+  if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
   for (auto &pair : ctx->ev)
     for (auto &e : pair)
       if (e) (void)hipEventDestroy(e);
@@ -3398,6 +3790,62 @@ int alpacc_test_force_timeout(alpacc_ctx *ctx, int on) {
   if (!ctx) return ALPACC_ERR_ARG;
   ctx->force_timeout = on != 0;
   return 0;
+}
+
+int alpacc_set_tuning(alpacc_ctx *ctx, const alpacc_tuning *t) {
+  if (!ctx || !t) return ALPACC_ERR_ARG;
+  if (t->graph_mode < 0 || t->graph_mode > 2 || t->lex_blocks_per_cu < 1 ||
+      t->lex_blocks_per_cu > 8 || t->keys_blocks_per_cu < 1 || t->keys_blocks_per_cu > 8 ||
+      t->search_max_blocks < 64 || t->search_max_blocks > (1u << 20) ||
+      t->brackets_per_kib < 1 || t->brackets_per_kib > (1u << 20) ||
+      (t->search_block_threads != 256 && t->search_block_threads != 512 &&
+       t->search_block_threads != 1024) ||
+      (t->lexer_tables_in_lds != 0 && t->lexer_tables_in_lds != 1)) {
+    ctx->err = "tuning value out of range";
+    return ALPACC_ERR_ARG;
+  }
+  if (t->graph_mode != ctx->graph_mode) {
+    ctx->drop_graph();
+    ctx->last_plan.clear();
+  }
+  ctx->graph_mode = t->graph_mode;
+  ctx->lex_bpc = t->lex_blocks_per_cu;
+  ctx->keys_bpc = t->keys_blocks_per_cu;
+  ctx->search_cap = t->search_max_blocks;
+  ctx->b_per_byte = t->brackets_per_kib / 1024.0;
+  ctx->search_bs = t->search_block_threads;
+  if ((t->lexer_tables_in_lds != 0) != (ctx->lex_tables_lds != 0) && ctx->has_lexer) {
+    ctx->lex_tables_lds = t->lexer_tables_in_lds != 0;
+    ctx->drop_graph();
+    ctx->last_plan.clear();
+    std::string err;
+    if (place_lexer_tables(ctx, err) != 0) {
+      ctx->err = err;
+      return ALPACC_ERR_HIP;
+    }
+  }
+  return 0;
+}
+
+int alpacc_get_tuning(alpacc_ctx *ctx, alpacc_tuning *t) {
+  if (!ctx || !t) return ALPACC_ERR_ARG;
+  t->graph_mode = ctx->graph_mode;
+  t->lex_blocks_per_cu = ctx->lex_bpc;
+  t->keys_blocks_per_cu = ctx->keys_bpc;
+  t->search_max_blocks = ctx->search_cap;
+  t->brackets_per_kib = (uint32_t)std::min(ctx->b_per_byte * 1024.0 + 0.5, (double)(1u << 20));
+  t->search_block_threads = ctx->search_bs;
+  t->lexer_tables_in_lds = ctx->lex_tables_lds;
+  t->graph_launches = ctx->graph_launches;
+  t->graph_captures = ctx->graph_captures;
+  return 0;
+}
+
+uint64_t alpacc_device_bytes(alpacc_ctx *ctx) {
+  if (!ctx) return 0;
+  uint64_t b = ctx->table_bytes;
+  for (const auto &s : ctx->scratch) b += s.second;
+  return b;
 }
 
 int alpacc_last_counts(alpacc_ctx *ctx, uint64_t *counts) {
@@ -3510,24 +3958,26 @@ int alpacc_parse(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *d_p
   (void)hipSetDevice(ctx->device);
   hipStream_t st = (hipStream_t)stream;
   const uint64_t tcap = n > 0 ? n : 1;
-  uint32_t *kinds = (uint32_t *)ctx->buf(B_TOK_KIND, tcap * 4);
+  // token kinds as bytes (labels have 7 bits): only the key lookups read them
+  uint8_t *kinds8 = (uint8_t *)ctx->buf(B_TOK_KIND, tcap + 16);
   uint32_t *starts = (uint32_t *)ctx->buf(B_TOK_START, tcap * 4);
   uint32_t *ends = (uint32_t *)ctx->buf(B_TOK_END, tcap * 4);
-  if (!kinds || !starts || !ends) {
+  if (!kinds8 || !starts || !ends) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
   // the parser starts on the lexer's device results; the lexer's verdict and token count come
   // back with the parser's first host synchronisation
   uint64_t T = 0;
-  int rc = run_lex(ctx, d_in, n, kinds, starts, ends, &T, st, 0, nullptr, nullptr, false);
+  int rc = run_lex(ctx, d_in, n, nullptr, starts, ends, &T, st, 0, nullptr, nullptr, false,
+                   nullptr, kinds8);
   if (rc != ALPACC_ACCEPT) {
     if (ctx->prof) ctx->prof_collect();
     return rc;
   }
   const int lex_launches = ctx->launches;
-  rc = run_parse_tokens<uint32_t>(ctx, kinds, starts, ends, n, d_parent, d_node, d_span, cap,
-                                  num_nodes, st, false, true);
+  rc = run_parse_tokens<uint32_t>(ctx, nullptr, starts, ends, n, d_parent, d_node, d_span, cap,
+                                  num_nodes, st, false, true, kinds8);
   ctx->launches += lex_launches;
   if (rc != ALPACC_ACCEPT) ctx->prof_drop(S_PARSE);
   if (ctx->prof) ctx->prof_collect();
@@ -4068,6 +4518,10 @@ void alpacc_event_destroy(void *event) {
 
 int alpacc_event_record(void *event, void *stream) {
   return hipEventRecord((hipEvent_t)event, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+int alpacc_event_sync(void *event) {
+  return hipEventSynchronize((hipEvent_t)event) == hipSuccess ? 0 : -1;
 }
 
 float alpacc_event_elapsed_ms(void *start, void *end) {

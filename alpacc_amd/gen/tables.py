@@ -9,7 +9,10 @@ Device layouts (all little-endian, row-major):
   * ``endo_apply``  u8[e*(m+1)]    DFA state reached by applying endo a to state s (0 = dead)
   * ``dfa_step``    u16[(m+1)*C]   next state | produce << 8 | label(next) << 9
   * ``state_info``  u8[m+1]        accept flag (bit 0)
-  * ``pair_index``  u32[NT^(q+k)]  dense LLP pair table: entry id or 0xFFFFFFFF
+  * ``pair_index``  u32[NT^(q+k)]  dense LLP pair table: entry id or 0xFFFFFFFF (up to 2^26 keys)
+  * ``pair_hash_keys`` u64[H], ``pair_hash_vals`` u32[H]: the pair table hashed instead (more
+                   than 2^26 possible windows): the window's dense index at its multiplicative
+                   hash slot (linear probing, H a power of two >= 2 * keys), entry id beside it
   * ``pair_entry``  u32[E*4]       (stack start, stack end, production start, production end)
   * ``stacks``      u16[]          brackets: symbol id | 0x8000 for a left (push) bracket
   * ``productions`` u16[]          production ids
@@ -31,6 +34,9 @@ from .llp import LLPBuilder, LLPTable
 
 MISSING = 0xFFFFFFFF
 LEFT_BIT = 0x8000
+DENSE_MAX = 1 << 26                 # largest dense pair index; beyond it the table is hashed
+HASH_EMPTY = 0xFFFFFFFFFFFFFFFF     # empty slot of the hashed pair table
+HASH_MUL = 0x9E3779B97F4A7C15       # multiplicative hash of a key's dense index
 
 
 @dataclass
@@ -67,7 +73,9 @@ class ParserTables:
     llp: LLPTable
     grammar: Grammar
     # device layouts
-    pair_index: np.ndarray
+    pair_index: np.ndarray           # dense (empty when hashed)
+    pair_hash_keys: np.ndarray       # hashed: u64 dense key index per slot (HASH_EMPTY: free)
+    pair_hash_vals: np.ndarray       # hashed: u32 entry id per slot
     pair_entry: np.ndarray
     stacks: np.ndarray
     productions: np.ndarray
@@ -162,7 +170,30 @@ def _fnv(key) -> int:
     return h
 
 
-def _build_parser(g: Grammar) -> ParserTables:
+def device_hash_slot(di: int, bits: int) -> int:
+    """Home slot of dense key index di in a hashed pair table of 2**bits slots."""
+    return ((di * HASH_MUL) & 0xFFFFFFFFFFFFFFFF) >> (64 - bits)
+
+
+def _device_hash(entries) -> tuple[np.ndarray, np.ndarray]:
+    """Open addressing with linear probing over (dense key index, entry id) pairs, at least twice
+    as many slots as keys (a power of two); the empty key is HASH_EMPTY."""
+    size = 16
+    while size < 2 * len(entries):
+        size *= 2
+    bits = size.bit_length() - 1
+    keys = np.full(size, HASH_EMPTY, dtype=np.uint64)
+    vals = np.full(size, MISSING, dtype=np.uint32)
+    for di, v in entries:
+        h = device_hash_slot(di, bits)
+        while keys[h] != np.uint64(HASH_EMPTY):
+            h = (h + 1) & (size - 1)
+        keys[h] = di
+        vals[h] = v
+    return keys, vals
+
+
+def _build_parser(g: Grammar, pair_table: str = "auto") -> ParserTables:
     q, k = g.q, g.k
     tb = LLPBuilder(g, q, k).build()
     empty = g.sym.empty
@@ -208,11 +239,18 @@ def _build_parser(g: Grammar) -> ParserTables:
                 max_iters = max(max_iters, p + 1)
                 break
             p += 1
-    # dense device table
+    # device pair table: dense over all nT^(q+k) windows, or hashed by the window's dense index
+    # when that would be too large (the reference always hashes: HashTable.hs:38-49)
     dense = nT ** (q + k)
-    if dense > (1 << 26):
+    if pair_table not in ("auto", "dense", "hash"):
+        raise ValueError(pair_table)
+    hashed = pair_table == "hash" or (pair_table == "auto" and dense > DENSE_MAX)
+    if not hashed and dense > DENSE_MAX:
         raise ValueError(f"dense pair table too large ({dense} entries)")
-    pair_index = np.full(dense, MISSING, dtype=np.uint32)
+    if hashed and dense >= HASH_EMPTY:
+        raise ValueError("pair table keys do not fit 64 bits")
+    pair_index = np.full(0 if hashed else dense, MISSING, dtype=np.uint32)
+    hentries = []
     entries = []
     stacks = []
     prods = []
@@ -230,7 +268,15 @@ def _build_parser(g: Grammar) -> ParserTables:
         di = 0
         for t in key:
             di = di * nT + t
-        pair_index[di] = uniq[sig]
+        if hashed:
+            hentries.append((di, uniq[sig]))
+        else:
+            pair_index[di] = uniq[sig]
+    if hashed:
+        pair_hash_keys, pair_hash_vals = _device_hash(hentries)
+    else:
+        pair_hash_keys = np.zeros(0, dtype=np.uint64)
+        pair_hash_vals = np.zeros(0, dtype=np.uint32)
     if nsym >= LEFT_BIT:
         raise ValueError("too many grammar symbols for 15-bit brackets")
     p2t = np.array([-1 if t is None else t for t in g.production_to_terminal], dtype=np.int32)
@@ -239,6 +285,7 @@ def _build_parser(g: Grammar) -> ParserTables:
         end_terminal=g.sym.end_terminal, empty_terminal=empty,
         production_names=list(g.production_names), arities=np.array(g.arities, dtype=np.int32),
         production_to_terminal=p2t, llp=tb, grammar=g, pair_index=pair_index,
+        pair_hash_keys=pair_hash_keys, pair_hash_vals=pair_hash_vals,
         pair_entry=np.array(entries, dtype=np.uint32).reshape(-1, 4),
         stacks=np.array(stacks, dtype=np.uint16), productions=np.array(prods, dtype=np.uint16),
         hash_keys=hkeys, hash_used=used, hash_vals=hvals, max_iters=max_iters,
@@ -246,10 +293,13 @@ def _build_parser(g: Grammar) -> ParserTables:
         bracket_bits=bracket_bits)
 
 
-def generate(source: str, mode: str = "both", name: str | None = None) -> Bundle:
+def generate(source: str, mode: str = "both", name: str | None = None,
+             pair_table: str = "auto") -> Bundle:
     """Build the bundle of a grammar (``mkLexer``/``mkParser``/``mkLexerParser``).
 
     ``source`` is a path to a ``.alp`` file or its text; ``mode`` is "lexer", "parser" or "both".
+    ``pair_table``: the device pair table is dense ("dense"), hashed ("hash"), or dense up to
+    DENSE_MAX windows and hashed beyond ("auto").
     """
     cfg = load_cfg(source)
     if name is None:
@@ -261,7 +311,7 @@ def generate(source: str, mode: str = "both", name: str | None = None) -> Bundle
         return Bundle(name=name, mode=mode, terminal_names=names, lexer=lex)
     g = build_grammar(cfg)
     names = terminal_names_both(g)
-    par = _build_parser(g)
+    par = _build_parser(g, pair_table)
     lex = None
     if mode == "both":
         ids = dict(g.sym.t_id)

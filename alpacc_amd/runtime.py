@@ -51,6 +51,8 @@ class AlpaccTables(ctypes.Structure):
         ("num_pair_entries", ctypes.c_uint32), ("pair_entry", _u32p),
         ("stacks_len", ctypes.c_uint32), ("stacks", _u16p),
         ("productions_len", ctypes.c_uint32), ("productions", _u16p),
+        ("pair_hash_len", ctypes.c_uint64), ("pair_hash_keys", ctypes.POINTER(ctypes.c_uint64)),
+        ("pair_hash_vals", _u32p),
     ]
 
 
@@ -63,7 +65,8 @@ EXPORTS = ("alpacc_create", "alpacc_create_error", "alpacc_destroy", "alpacc_las
            "alpacc_event_elapsed_ms", "alpacc_profile_enable", "alpacc_profile_read",
            "alpacc_lex_chunked", "alpacc_parse_chunked", "alpacc_lex_batch", "alpacc_parse_batch",
            "alpacc_last_counts", "alpacc_host_alloc", "alpacc_host_free", "alpacc_copy_device",
-           "alpacc_test", "alpacc_free", "alpacc_test_force_timeout", "alpacc_lex_noalloc")
+           "alpacc_test", "alpacc_free", "alpacc_test_force_timeout", "alpacc_lex_noalloc",
+           "alpacc_set_tuning", "alpacc_get_tuning", "alpacc_event_sync", "alpacc_device_bytes")
 
 # the chunk size of the reference's generated entries (Generator.hs:24-76: lexer.lex_int 16777216)
 REFERENCE_CHUNK = 16777216
@@ -97,6 +100,10 @@ def load_library():
     L.alpacc_lex_chunked.argtypes = [vp, vp, u64, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
     L.alpacc_parse_chunked.argtypes = [vp, vp, u64, u64, vp, vp, vp, u64, ctypes.POINTER(u64), vp]
     L.alpacc_last_counts.argtypes = [vp, vp]
+    L.alpacc_device_bytes.argtypes = [vp]
+    L.alpacc_device_bytes.restype = u64
+    L.alpacc_set_tuning.argtypes = [vp, ctypes.POINTER(AlpaccTuning)]
+    L.alpacc_get_tuning.argtypes = [vp, ctypes.POINTER(AlpaccTuning)]
     L.alpacc_copy_device.argtypes = [vp, vp, u64, vp]
     L.alpacc_test.argtypes = [vp, vp, u64, ctypes.POINTER(vp), ctypes.POINTER(u64), vp]
     L.alpacc_free.argtypes = [vp]
@@ -120,6 +127,7 @@ def load_library():
     L.alpacc_event_create.restype = vp
     L.alpacc_event_destroy.argtypes = [vp]
     L.alpacc_event_record.argtypes = [vp, vp]
+    L.alpacc_event_sync.argtypes = [vp]
     L.alpacc_event_elapsed_ms.argtypes = [vp, vp]
     L.alpacc_event_elapsed_ms.restype = ctypes.c_float
     L.alpacc_profile_enable.argtypes = [vp, ctypes.c_int]
@@ -127,6 +135,19 @@ def load_library():
                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64), ctypes.c_int]
     _lib = L
     return L
+
+
+class AlpaccTuning(ctypes.Structure):
+    """alpacc_tuning (include/alpacc_hip.h): a context's launch choices."""
+    _fields_ = [("graph_mode", ctypes.c_int32), ("lex_blocks_per_cu", ctypes.c_uint32),
+                ("keys_blocks_per_cu", ctypes.c_uint32), ("search_max_blocks", ctypes.c_uint32),
+                ("brackets_per_kib", ctypes.c_uint32), ("search_block_threads", ctypes.c_uint32),
+                ("lexer_tables_in_lds", ctypes.c_int32), ("reserved", ctypes.c_uint32),
+                ("graph_launches", ctypes.c_uint64), ("graph_captures", ctypes.c_uint64)]
+
+
+TUNING_KEYS = ("graph_mode", "lex_blocks_per_cu", "keys_blocks_per_cu", "search_max_blocks",
+               "brackets_per_kib", "search_block_threads", "lexer_tables_in_lds")
 
 
 # alpacc_doc_result: status, reserved, offset, count
@@ -194,6 +215,90 @@ def copy_into(dst: np.ndarray, src: np.ndarray) -> None:
     cuts = list(range(0, n, COPY_SPLIT)) + [n]
     list(_COPY_POOL.map(lambda ab: np.copyto(dst[ab[0]:ab[1]], src[ab[0]:ab[1]]),
                         zip(cuts[:-1], cuts[1:])))
+
+
+def _copy_pool() -> ThreadPoolExecutor:
+    global _COPY_POOL
+    with _COPY_POOL_LOCK:
+        if _COPY_POOL is None:
+            _COPY_POOL = ThreadPoolExecutor(max_workers=8)
+    return _COPY_POOL
+
+
+class SliceEvents:
+    """HIP events for the slices of one pipelined transfer (created once, reused)."""
+
+    def __init__(self, lib, count: int = 64):
+        self.lib = lib
+        self.events = [lib.alpacc_event_create() for _ in range(count)]
+        if not all(self.events):
+            self.free()
+            raise RuntimeError("event creation failed")
+
+    def free(self):
+        for e in self.events or []:
+            if e:
+                self.lib.alpacc_event_destroy(ctypes.c_void_p(e))
+        self.events = []
+
+
+def _slices(parts, nslices_max: int):
+    """(device address, bytes, host offset) parts cut into at most nslices_max slices of at least
+    COPY_SPLIT bytes, in order."""
+    total = sum(nb for _, nb, _ in parts)
+    size = max(COPY_SPLIT, -(-total // max(1, nslices_max)))
+    size = (size + 4095) & ~4095
+    out = []
+    for dev, nb, hoff in parts:
+        o = 0
+        while o < nb:
+            k = min(size, nb - o)
+            out.append((dev + o, k, hoff + o))
+            o += k
+    return out
+
+
+def pipelined_download(lib, stream, parts, pin, dst: np.ndarray, events: SliceEvents) -> None:
+    """Device ranges into the uint8 host array dst through the pinned buffer pin (same offsets):
+    every slice is DMA'd into pin and its event recorded at once; the host copies each slice out
+    of pin as soon as its event completes, so the DMA of later slices overlaps the host copies of
+    earlier ones. parts: (device address, bytes, offset in dst)."""
+    # at most (slice count asked for) + (number of parts) slices: one event each
+    sl = _slices(parts, max(1, len(events.events) - len(parts)))
+    if len(sl) > len(events.events):
+        raise ValueError("too many parts for the transfer's events")
+    for i, (dev, nb, off) in enumerate(sl):
+        if lib.alpacc_copy_to_host(ctypes.c_void_p(pin.ptr + off), ctypes.c_void_p(dev), nb,
+                                   stream) != 0 or \
+                lib.alpacc_event_record(ctypes.c_void_p(events.events[i]), stream) != 0:
+            raise RuntimeError("device to host copy failed")
+    pool = _copy_pool()
+    futs = []
+    for i, (_, nb, off) in enumerate(sl):
+        if lib.alpacc_event_sync(ctypes.c_void_p(events.events[i])) != 0:
+            raise RuntimeError("device to host copy failed")
+        futs.append(pool.submit(np.copyto, dst[off:off + nb], pin.array[off:off + nb]))
+    for f in futs:
+        f.result()
+
+
+def pipelined_upload(lib, stream, srcs, pin, dev: int) -> None:
+    """uint8 host arrays to the device buffer at address dev through the pinned buffer pin (same
+    offsets): a pool of threads copies slices into pin, and each slice's DMA is queued as soon as
+    it is there. srcs: an array (at offset 0) or a list of (array, offset)."""
+    if isinstance(srcs, np.ndarray):
+        srcs = [(srcs, 0)]
+    sl = []
+    for src, base in srcs:
+        for _, nb, off in _slices([(0, int(src.shape[0]), 0)], 64):
+            sl.append((src[off:off + nb], base + off, nb))
+    pool = _copy_pool()
+    futs = [pool.submit(np.copyto, pin.array[o:o + nb], part) for part, o, nb in sl]
+    for (_, o, nb), f in zip(sl, futs):
+        f.result()
+        if nb and lib.alpacc_copy_to_device(ctypes.c_void_p(dev + o), ctypes.c_void_p(pin.ptr + o),
+                                            nb, stream) != 0:
+            raise RuntimeError("host to device copy failed")
 
 
 class HostBuffer:
@@ -264,7 +369,13 @@ def fill_tables(bundle: Bundle) -> tuple[AlpaccTables, list]:
         t.arities = _ptr(hold(px.arities, np.int32), _i32p)
         t.prod_to_terminal = _ptr(hold(px.production_to_terminal, np.int32), _i32p)
         t.pair_index_len = px.pair_index.shape[0]
-        t.pair_index = _ptr(hold(px.pair_index, np.uint32), _u32p)
+        t.pair_index = _ptr(hold(px.pair_index, np.uint32), _u32p) if px.pair_index.shape[0] \
+            else ctypes.cast(None, _u32p)
+        t.pair_hash_len = px.pair_hash_keys.shape[0]
+        if px.pair_hash_keys.shape[0]:
+            t.pair_hash_keys = _ptr(hold(px.pair_hash_keys, np.uint64),
+                                    ctypes.POINTER(ctypes.c_uint64))
+            t.pair_hash_vals = _ptr(hold(px.pair_hash_vals, np.uint32), _u32p)
         t.num_pair_entries = px.pair_entry.shape[0]
         t.pair_entry = _ptr(hold(px.pair_entry.reshape(-1), np.uint32), _u32p)
         t.stacks_len = px.stacks.shape[0]
@@ -291,12 +402,16 @@ class Context:
         self.stream = None          # default stream
         self._bufs: dict = {}
         self._pins: dict = {}  # grow-only pinned staging for large transfers
+        self._events = None    # events of pipelined downloads
 
     def close(self):
         for b in list(self._bufs.values()) + list(self._pins.values()):
             b.free()
         self._bufs.clear()
         self._pins.clear()
+        if getattr(self, "_events", None) is not None:
+            self._events.free()
+            self._events = None
         if getattr(self, "handle", None):
             self.lib.alpacc_destroy(self.handle)
             self.handle = None
@@ -338,11 +453,7 @@ class Context:
         n = int(arr.shape[0])
         buf = self.buffer("doc", n)
         if n >= PINNED_MIN:  # through pinned memory: the device copy runs at DMA speed
-            pin = self.pinned("doc", n)
-            copy_into(pin.array[:n], arr)
-            if self.lib.alpacc_copy_to_device(ctypes.c_void_p(buf.ptr), ctypes.c_void_p(pin.ptr),
-                                              n, self.stream) != 0:
-                raise RuntimeError("host to device copy failed")
+            pipelined_upload(self.lib, self.stream, arr, self.pinned("doc", n), buf.ptr)
         elif n:
             buf.upload(arr, self.stream)
         return buf, n
@@ -358,12 +469,12 @@ class Context:
         if sum(sizes) < PINNED_MIN:
             return [b.download(dt, int(c), self.stream) for b, dt, c in parts]
         pin = self.pinned("out", total)
-        for (b, _, _), size, off in zip(parts, sizes, offs):
-            if size and self.lib.alpacc_copy_to_host(ctypes.c_void_p(pin.ptr + off),
-                                                     ctypes.c_void_p(b.ptr), size, self.stream) != 0:
-                raise RuntimeError("device to host copy failed")
+        if self._events is None:
+            self._events = SliceEvents(self.lib)
         mem = np.empty(total, dtype=np.uint8)
-        copy_into(mem, pin.array[:total])
+        pipelined_download(self.lib, self.stream,
+                           [(b.ptr, size, off) for (b, _, _), size, off in zip(parts, sizes, offs)
+                            if size], pin, mem, self._events)
         return [mem[off:off + size].view(dt) for (_, dt, _), size, off in zip(parts, sizes, offs)]
 
     def download_tree(self, pb: DeviceBuffer, nb: DeviceBuffer, spb: DeviceBuffer, nn: int):
@@ -371,6 +482,27 @@ class Context:
         par, node, span = self.download_arrays(((pb, np.uint32, nn), (nb, np.uint32, nn),
                                                 (spb, np.uint32, 2 * nn)))
         return par, node, span.reshape(-1, 2)
+
+    # -- launch choices ----------------------------------------------------------------------
+    def tuning(self) -> dict:
+        """The context's launch choices and graph counters (alpacc_get_tuning)."""
+        t = AlpaccTuning()
+        if self.lib.alpacc_get_tuning(self.handle, ctypes.byref(t)) != 0:
+            raise self._err(-1)
+        return {name: int(getattr(t, name)) for name, _ in AlpaccTuning._fields_}
+
+    def set_tuning(self, **kw) -> dict:
+        """Change some launch choices (alpacc_set_tuning); returns the new ones."""
+        cur = self.tuning()
+        for k, v in kw.items():
+            if k not in TUNING_KEYS:
+                raise KeyError(k)
+            cur[k] = int(v)
+        t = AlpaccTuning(*(cur[name] for name, _ in AlpaccTuning._fields_))
+        rc = self.lib.alpacc_set_tuning(self.handle, ctypes.byref(t))
+        if rc != 0:
+            raise self._err(rc)
+        return self.tuning()
 
     # -- profiling ---------------------------------------------------------------------------
     def profile(self, on: bool = True):
@@ -386,6 +518,11 @@ class Context:
         calls = (ctypes.c_uint64 * m)()
         k = self.lib.alpacc_profile_read(self.handle, names, ms, calls, m)
         return {names[i].decode(): (ms[i], calls[i]) for i in range(k)}
+
+    def device_bytes(self) -> int:
+        """Device memory of the context (tables and scratch arenas) plus its Python-side buffers."""
+        return int(self.lib.alpacc_device_bytes(self.handle)) + sum(
+            b.nbytes for b in self._bufs.values())
 
     def last_counts(self) -> dict:
         """Element counts of the last parser call: tokens, positions, brackets, productions."""

@@ -9,10 +9,12 @@ and serves a stream of documents:
   copy into pinned memory runs at DMA speed; the fresh per-request arrays are then filled by a
   pool of host threads);
 * a worker that becomes free takes the requests waiting at that moment, in arrival order, as long
-  as their estimated HBM footprint fits the shared budget (admission by footprint) and the batch
-  limits hold, and runs them as one ``alpacc_parse_batch`` call. A batch holds requests of one
-  size class (below ``large_bytes``, or from it up): small requests are not held up by a large
-  one's copies;
+  as the batch limits hold and the device memory fits the shared budget (admission by footprint):
+  every worker's measured holdings (context tables and arenas, its buffers), the growth reserved by
+  running batches larger than their worker is sized for, and this batch's growth at the measured
+  device bytes per input byte; it runs them as one ``alpacc_parse_batch`` call. A batch holds
+  requests of one size class (below ``large_bytes``, or from it up): small requests are not held
+  up by a large one's copies;
 * with ``warmup`` (a document as large as the largest batch), every worker runs it once before
   serving, so that its device arenas, its pinned staging and its output buffers already have their
   final sizes: growing them while serving frees device memory, and a free waits for every stream
@@ -33,20 +35,17 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .runtime import ACCEPT, Context, DeviceBuffer, HostBuffer, copy_into, load_library
+from .runtime import (ACCEPT, Context, DeviceBuffer, HostBuffer, SliceEvents, load_library,
+                      pipelined_download, pipelined_upload)
 
-# Device bytes per input byte while a request is in flight: the input, the token slots
-# (3 x u32 per byte), key positions and pair entries, brackets / productions with their scan
-# values and minima, and the tree (u32 parent, node, span x2) at up to one node per byte.
+# Device bytes per input byte of a worker sized for a batch, before any batch has measured it:
+# the input, the token slots (3 x u32 per byte), key positions and pair entries, brackets /
+# productions with their scan values and minima, and the tree (u32 parent, node, span x2) at up to
+# one node per byte. Workers replace it with what their allocations hold (Server._per_byte).
 FOOTPRINT_PER_BYTE = 48
-FOOTPRINT_FIXED = 1 << 20
 
 
 _CREATE_LOCK = threading.Lock()  # alpacc_create reports errors through one global string
-
-
-def footprint(nbytes: int) -> int:
-    return FOOTPRINT_PER_BYTE * nbytes + FOOTPRINT_FIXED
 
 
 @dataclass
@@ -87,6 +86,13 @@ class Server:
         self.device = device
         self._cv = threading.Condition()
         self._pending: deque = deque()
+        # admission by measured device memory: what every worker holds (context tables and arenas,
+        # its input and output buffers), the largest batch each is sized for, the device bytes
+        # per input byte of a worker so sized (measured), and the growth reserved by running
+        # batches that exceed their worker's size
+        self._resident = [0] * workers
+        self._sized_for = [0] * workers
+        self._per_byte = float(FOOTPRINT_PER_BYTE)
         self._inflight = 0
         self._closing = False
         self._next_id = 0
@@ -109,7 +115,7 @@ class Server:
     def submit(self, data, arrival: float | None = None) -> Request:
         arr = np.frombuffer(bytes(data), dtype=np.uint8) if isinstance(data, (bytes, bytearray)) \
             else np.ascontiguousarray(data, dtype=np.uint8)
-        if footprint(arr.shape[0]) > self.budget:
+        if self._per_byte * arr.shape[0] > self.budget:
             raise ValueError("request larger than the HBM budget")
         with self._cv:
             r = Request(self._next_id, arr, time.perf_counter() if arrival is None else arrival)
@@ -126,41 +132,65 @@ class Server:
             t.join()
 
     # -- worker side ----------------------------------------------------------------------------
-    def _admit(self) -> list:
-        """Requests waiting now, in arrival order, of the first one's size class, while the
-        footprint and batch limits hold."""
+    def _growth(self, w: int, nbytes: int) -> int:
+        """Device bytes worker w must add to run a batch of nbytes input bytes."""
+        return int(max(0, nbytes - self._sized_for[w]) * self._per_byte)
+
+    def _admit(self, w: int = 0) -> tuple[list, int]:
+        """Requests waiting now, in arrival order, of the first one's size class, while the batch
+        limits hold and the device memory fits the budget: every worker's measured holdings, the
+        growth reserved by running batches, and the growth this batch needs on worker w. Returns
+        the batch and the growth it reserves."""
         batch, nbytes, skipped = [], 0, []
         large = None
+        held = sum(self._resident) + self._inflight
         while self._pending and len(batch) < self.max_docs:
             r = self._pending[0]
             n = r.data.shape[0]
             if large is not None and (n >= self.large) != large:
                 skipped.append(self._pending.popleft())  # the other class: left for later
                 continue
-            fp = footprint(n)
-            if batch and (nbytes + n > self.max_bytes or self._inflight + fp > self.budget):
+            # a batch within its worker's size allocates nothing and always fits
+            g = self._growth(w, nbytes + n)
+            over = g > 0 and held + g > self.budget
+            if batch and (nbytes + n > self.max_bytes or over):
                 break
-            if not batch and self._inflight + fp > self.budget:
+            if not batch and over:
                 break  # wait for memory to free up
             self._pending.popleft()
             batch.append(r)
             large = n >= self.large
             nbytes += n
-            self._inflight += fp
         self._pending.extendleft(reversed(skipped))  # back in arrival order
-        return batch
+        grow = self._growth(w, nbytes) if batch else 0
+        self._inflight += grow
+        return batch, grow
+
+    def _measured(self, w: int, held: int, nbytes: int) -> None:
+        """Worker w holds `held` device bytes after a batch of nbytes input bytes (lock held)."""
+        self._resident[w] = held
+        self._sized_for[w] = max(self._sized_for[w], nbytes)
+        per = [self._resident[i] / self._sized_for[i] for i in range(len(self._resident))
+               if self._sized_for[i] >= (1 << 20)]
+        if per:
+            self._per_byte = max(per)
 
     def _worker(self, w: int):
         ctx = stream = None
-        state = {"in": None, "pin_in": None, "out": None, "pin_out": None, "ratio": 1.0}
+        state = {"in": None, "pin_in": None, "out": None, "pin_out": None, "ratio": 1.0,
+                 "events": None}
         try:
             lib = load_library()
             with _CREATE_LOCK:
                 ctx = Context(self.bundle, self.device)
             stream = lib.alpacc_stream_create()
             ctx.stream = stream
+            state["events"] = SliceEvents(lib)
             if self._warm is not None:
                 self._run(ctx, lib, stream, state, [Request(-1, self._warm)], -1)
+            held = self._held(ctx, state)
+            with self._cv:
+                self._measured(w, held, 0 if self._warm is None else self._warm.shape[0])
         except Exception as exc:  # reported to the constructor
             self._errors.append(exc)
             self._release(ctx, stream, state)
@@ -173,7 +203,7 @@ class Server:
                     self._cv.wait()
                 if not self._pending and self._closing:
                     break
-                batch = self._admit()
+                batch, grow = self._admit(w)
                 if not batch:
                     self._cv.wait(0.001)
                     continue
@@ -191,10 +221,18 @@ class Server:
                         r.t_done = now
                         r.done.set()
             finally:
+                held = self._held(ctx, state)
                 with self._cv:
-                    self._inflight -= sum(footprint(r.data.shape[0]) for r in batch)
+                    self._inflight -= grow
+                    self._measured(w, held, sum(r.data.shape[0] for r in batch))
                     self._cv.notify_all()
         self._release(ctx, stream, state)
+
+    @staticmethod
+    def _held(ctx, state) -> int:
+        """Device bytes a worker holds: its context's tables and arenas, its input and output
+        buffers."""
+        return ctx.device_bytes() + sum(b.nbytes for k in ("in", "out") for b in (state[k] or []))
 
     @staticmethod
     def _release(ctx, stream, state):
@@ -203,7 +241,7 @@ class Server:
             for b in (state[k] or []):
                 b.free()
             state[k] = None
-        for k in ("pin_in", "pin_out"):
+        for k in ("pin_in", "pin_out", "events"):
             if state[k] is not None:
                 state[k].free()
             state[k] = None
@@ -238,12 +276,12 @@ class Server:
             offsets[1:] = np.cumsum(lengths[:-1])
         total = int(lengths.sum())
         pin = self._grow_pin(state, "pin_in", total, lib)
-        for r, o in zip(batch, offsets.tolist()):
-            copy_into(pin.array[o:o + r.data.shape[0]], r.data)
-        t_staged = time.perf_counter()
         (d_in,) = self._grow_dev(state, "in", [total], lib)
-        if total and lib.alpacc_copy_to_device(d_in.ptr, pin.ptr, total, stream) != 0:
-            raise RuntimeError("host to device copy failed")
+        # the documents go in through pinned memory, each slice's DMA queued as soon as the host
+        # has copied it there
+        pipelined_upload(lib, stream, [(r.data, o) for r, o in zip(batch, offsets.tolist())],
+                         pin, d_in.ptr)
+        t_staged = time.perf_counter()
         cap = max(16, int(total * state["ratio"] * 1.25) + 16)
         while True:
             outs = self._grow_dev(state, "out", [4 * cap, 4 * cap, 8 * cap], lib)
@@ -256,16 +294,15 @@ class Server:
         t_parsed = time.perf_counter()
         if total:
             state["ratio"] = max(state["ratio"] * 0.9, nodes / total)
-        # the batch's trees come back through pinned memory into fresh arrays; every request gets
-        # views of its range
+        # the batch's trees come back through pinned memory into fresh arrays, in slices whose
+        # DMA overlaps the host copies of the slices before; every request gets views of its range
         pin = self._grow_pin(state, "pin_out", 16 * nodes, lib)
-        if nodes:
-            for off, b, nb in ((0, outs[0], 4 * nodes), (4 * nodes, outs[1], 4 * nodes),
-                               (8 * nodes, outs[2], 8 * nodes)):
-                if lib.alpacc_copy_to_host(pin.ptr + off, b.ptr, nb, stream) != 0:
-                    raise RuntimeError("device to host copy failed")
         mem = np.empty(16 * nodes, dtype=np.uint8)
-        copy_into(mem, pin.array[:16 * nodes])
+        if nodes:
+            pipelined_download(lib, stream, [(outs[0].ptr, 4 * nodes, 0),
+                                             (outs[1].ptr, 4 * nodes, 4 * nodes),
+                                             (outs[2].ptr, 8 * nodes, 8 * nodes)],
+                               pin, mem, state["events"])
         par = mem[:4 * nodes].view(np.uint32)
         nd = mem[4 * nodes:8 * nodes].view(np.uint32)
         sp = mem[8 * nodes:].view(np.uint32).reshape(-1, 2)

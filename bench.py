@@ -35,7 +35,8 @@ def parse_args():
     ap.add_argument("--mib", type=int, default=256, help="document size in MiB")
     ap.add_argument("--workload", choices=["parse", "lex"], default="parse")
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--cpu-sample-mib", type=float, default=16.0)
+    ap.add_argument("--cpu-sample-mib", type=float, default=16.0,
+                    help="size of the one-thread CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verify", action="store_true", help="check the tree against the C oracle")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sexp comparison")
@@ -58,28 +59,71 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(bundle, workload, mib, seed):
-    """Time the C/OpenMP oracle on a bounded sample generated like the benchmark document."""
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _median_time(fn, reps: int = 5) -> float:
+    """Median wall time of `reps` calls after one warm-up call."""
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(sorted(ts)[len(ts) // 2])
+
+
+def cpu_baseline(bundle, workload, doc, seed, one_thread_mib):
+    """The C/OpenMP restatement of lexer.fut + parser.fut (the oracle) timed on this host's cores:
+    the benchmark document itself in mode U (unchunked) and mode F (the 16 MiB chunk loop of
+    lexer.fut:144-172, as the generated entries run it), on all threads and, on a bounded sample,
+    on one thread; plus the CPU-only arithmetic workload (1 MiB, seed 42). Every figure is the
+    median of 5 runs after one warm-up, input already in memory, output allocation included."""
     from alpacc_amd import datagen
+    from alpacc_amd.gen.tables import generate
     from oracle.c import coracle
-    n = int(mib * (1 << 20))
-    doc = datagen.json_doc(n, seed=seed + 1)
+    n = int(doc.shape[0])
     orc = coracle.Oracle(bundle)
-    threads = os.cpu_count() or 1
+    threads = orc.threads
     fn = orc.parse if workload == "parse" else orc.lex
-    fn(doc)  # warm-up
-    reps = 0
-    t0 = time.perf_counter()
-    while True:
-        fn(doc)
-        reps += 1
-        if time.perf_counter() - t0 > 10.0 or reps >= 20:
-            break
-    dt = (time.perf_counter() - t0) / reps
-    return {"value": round(n / dt / 1e9, 4), "unit": "GB/s", "cores": orc.threads or threads,
-            "kind": "port",
-            "sample": f"{mib:g} MiB seeded JSON document (seed {seed + 1}), {workload} with the "
-                      f"C/OpenMP restatement of lexer.fut+parser.fut, mean of {reps} runs"}
+    chunk = 16 << 20
+    t_u = _median_time(lambda: fn(doc))
+    t_f = _median_time(lambda: fn(doc, chunk=chunk))
+    small = datagen.json_doc(int(one_thread_mib * (1 << 20)), seed=seed)
+    arith_b = generate(os.path.join(ROOT, "alpacc_amd", "grammars", "arithmetic.alp"))
+    arith = datagen.arith_doc(1 << 20, seed=seed)
+    aorc = coracle.Oracle(arith_b)
+    t_a = _median_time(lambda: aorc.parse(arith))
+    orc.set_threads(1)
+    try:
+        t_1 = _median_time(lambda: fn(small))
+        t_a1 = _median_time(lambda: aorc.parse(arith))
+    finally:
+        orc.set_threads(threads)
+    gbs = lambda nb, t: round(nb / t / 1e9, 4)  # noqa: E731
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return {"value": gbs(n, t_u), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"the benchmark document itself ({n >> 20} MiB JSON, seed {seed}), "
+                      f"{workload} with the C/OpenMP restatement of lexer.fut+parser.fut (mode U, "
+                      f"unchunked), {threads} threads, median of 5 runs after one warm-up",
+            "cpu_model": _cpu_model(), "nproc": cores,
+            "mode_F_GBps": gbs(n, t_f),
+            "one_thread": {"GBps": gbs(small.shape[0], t_1),
+                           "sample": f"{one_thread_mib:g} MiB JSON document, seed {seed}, mode U"},
+            "arithmetic_1MiB": {"all_threads_GBps": gbs(arith.shape[0], t_a),
+                                "one_thread_GBps": gbs(arith.shape[0], t_a1),
+                                "ms_all_threads": round(t_a * 1e3, 2),
+                                "workload": "grammars/arithmetic.alp, 1 MiB, seed 42, lex+parse"}}
 
 
 def batched(args, lib):
@@ -101,7 +145,16 @@ def batched(args, lib):
         offsets[i] = off
         off += int(sz)
     lengths = sizes.astype(np.uint64)
+    # launch choices for this grammar, timed on the first 512 documents (about a quarter of the
+    # bytes) by the autotuner
+    from alpacc_amd import autotune
+    log("autotuning the batched launch choices")
+    sample = [datagen.sexp_doc(int(sz), seed=args.seed + 1000 + i)
+              for i, sz in enumerate(sizes[:512].tolist())]
+    tuned = autotune.tune(bundle, sample, reps=3, save_result=False)
+    del sample
     ctx = Context(bundle)
+    ctx.set_tuning(**tuned["params"])
     # size the outputs with one call (it reports the node count when the capacity is too small)
     rc, need, res = ctx.parse_batch_device(d_in.ptr, offsets, lengths, 0, 0, 0, 0)
     cap = int(need)
@@ -111,17 +164,35 @@ def batched(args, lib):
         return ctx.parse_batch_device(d_in.ptr, offsets, lengths, outs[0].ptr, outs[1].ptr,
                                       outs[2].ptr, cap)
 
+    def timed(steps):
+        lib.alpacc_stream_sync(None)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            rc_s, got_s, res_s = batch_step()
+            if rc_s != 1 or got_s != got or not np.array_equal(res_s, res):
+                raise SystemExit("batched results differ between calls")
+        lib.alpacc_stream_sync(None)
+        return (time.perf_counter() - t0) / steps
+
+    # queued directly: every kernel launched from the host, one synchronisation at the end
+    ctx.set_tuning(graph_mode=0)
+    batch_step()
     rc, got, res = batch_step()
     if rc != 1:
         raise SystemExit(f"batched call failed (rc={rc})")
     accepted = int((res["status"] == 1).sum())
+    t_direct = timed(args.batch_steps)
+    # the same call replayed as one HIP graph launch (captured when the plan repeats)
+    ctx.set_tuning(graph_mode=1)
+    batch_step()
+    batch_step()
+    g0 = ctx.tuning()["graph_launches"]
+    t_batch = timed(args.batch_steps)
+    tune = ctx.tuning()
+    graph_replays = tune["graph_launches"] - g0
+    # stage times (HIP events around every stage: queued directly)
     ctx.profile(True)
-    lib.alpacc_stream_sync(None)
-    t0 = time.perf_counter()
-    for _ in range(args.batch_steps):
-        batch_step()
-    lib.alpacc_stream_sync(None)
-    t_batch = (time.perf_counter() - t0) / args.batch_steps
+    batch_step()
     prof = ctx.read_profile()
     ctx.profile(False)
     # plain: one alpacc_parse call per document, same documents and output buffers
@@ -139,9 +210,16 @@ def batched(args, lib):
     if plain_acc != accepted or plain_nodes != int(res["count"].sum()):
         raise SystemExit("batched and per-document results disagree")
     out = {"workload": f"sexp lex+parse, {len(sizes)} documents, {total >> 20} MiB, batched "
-                       "(alpacc_parse_batch) vs one alpacc_parse per document",
+                       "(alpacc_parse_batch replayed as one HIP graph launch) vs one "
+                       "alpacc_parse per document",
            "docs": len(sizes), "bytes": total, "accepted": accepted, "nodes": int(got),
            "batched_ms": round(t_batch * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
+           "batched_direct_ms": round(t_direct * 1e3, 3),
+           "graph": {"replays_timed": int(graph_replays), "captures": int(tune["graph_captures"]),
+                     "vs_direct": round(t_direct / t_batch, 3)},
+           "autotuned": {"params": {k: tune[k] for k in autotune.DEFAULTS if k != "graph_mode"},
+                         "sample": f"first 512 documents ({tuned['bytes'] >> 20} MiB)",
+                         "sample_ms_default": tuned["default_ms"], "sample_ms_tuned": tuned["ms"]},
            "batched_docs_per_s": round(len(sizes) / t_batch, 1),
            "plain_docs_per_s": round(len(sizes) / t_plain, 1),
            "batched_GBps": round(total / t_batch / 1e9, 3),
@@ -350,7 +428,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(bundle, args.workload, args.cpu_sample_mib, args.seed)
+            log("cpu baseline")
+            cpu = cpu_baseline(bundle, args.workload, doc, args.seed, args.cpu_sample_mib)
         except Exception as exc:  # the baseline is reported, never required
             log(f"cpu baseline unavailable: {exc}")
     if not args.no_batch or not args.no_serve:

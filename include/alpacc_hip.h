@@ -38,7 +38,8 @@ extern "C" {
 
 /* The generated tables of one grammar (the reference's Analyzer record, Analyzer.hs:33-82),
  * in the device layouts described in alpacc_amd/gen/tables.py. Host pointers; alpacc_create
- * copies them. Parser fields may be zero/NULL for a lexer-only bundle. */
+ * copies them. Parser fields may be zero/NULL for a lexer-only bundle. A caller compiled against
+ * an older copy of this header passes a shorter struct: rebuild callers with the header. */
 typedef struct alpacc_tables {
   /* lexer (Analyzer.hs:33-41, futhark/lexer.fut:8-27) */
   uint32_t num_endos;        /* e: endomorphisms (identity = 0, dead = 1) */
@@ -60,7 +61,7 @@ typedef struct alpacc_tables {
   uint32_t num_productions;
   const int32_t *arities;          /* [P] */
   const int32_t *prod_to_terminal; /* [P], -1 = not a terminal production */
-  uint64_t pair_index_len;         /* num_terminals^(q+k) */
+  uint64_t pair_index_len;         /* num_terminals^(q+k), or 0 when the table is hashed */
   const uint32_t *pair_index;      /* dense pair table: entry id or 0xFFFFFFFF */
   uint32_t num_pair_entries;
   const uint32_t *pair_entry;      /* [E*4] stack start/end, production start/end */
@@ -68,6 +69,14 @@ typedef struct alpacc_tables {
   const uint16_t *stacks;          /* brackets: symbol | 0x8000 for left */
   uint32_t productions_len;
   const uint16_t *productions;
+  /* the pair table hashed instead of dense (pair_index_len 0, pair_index NULL), for grammars with
+   * more than 2^26 possible key windows: pair_hash_len slots (a power of two), each the dense
+   * index of a window (sum of terminal * num_terminals^position, first terminal most significant)
+   * or 0xFFFFFFFFFFFFFFFF when free, and its entry id; a window's home slot is
+   * (index * 0x9E3779B97F4A7C15 mod 2^64) >> (64 - log2(pair_hash_len)), linear probing. */
+  uint64_t pair_hash_len;
+  const uint64_t *pair_hash_keys;
+  const uint32_t *pair_hash_vals;
 } alpacc_tables;
 
 typedef struct alpacc_ctx alpacc_ctx;
@@ -187,6 +196,46 @@ int alpacc_last_launch_count(alpacc_ctx *ctx);
  * must then return ALPACC_ERR_TIMEOUT instead of results built on a wrong prefix. */
 int alpacc_test_force_timeout(alpacc_ctx *ctx, int on);
 
+/* Launch choices of a context (no counterpart in the reference, whose generated code fixes them
+ * at generation time: cuda/lexer.cu:1-48 derives its items per thread from the shared memory per
+ * block). alpacc_amd/autotune.py times candidates per grammar and stores the best.
+ *   graph_mode        : batched lex + parse (alpacc_parse_batch). 0 = queue the kernels directly;
+ *                       1 (default) = capture the call's queue into a HIP graph when its plan
+ *                       (sizes, buffers, flags) repeats the previous call's, and replay that graph
+ *                       while it keeps repeating; 2 = capture every new plan.
+ *   lex_blocks_per_cu : persistent lex_reduce workgroups per CU (1..8, default 2)
+ *   keys_blocks_per_cu: persistent parse_keys2 workgroups per CU (1..8, default 2)
+ *   search_max_blocks : largest grid of bracket_match / parse_parents, in workgroups of 256
+ *                       threads (64..2^20, default 65536)
+ *   search_block_threads: workgroup size of bracket_match / parse_parents (256, 512 or 1024;
+ *                       default 256)
+ *   brackets_per_kib  : brackets per KiB of input the batched bracket scratch is first sized for
+ *                       (1..2^20, default 256); a batch with more runs again sized exactly, and
+ *                       the estimate grows to what calls needed
+ *   lexer_tables_in_lds: 1 (default) = the lexer's tables in LDS where they fit (the
+ *                       (endo x class) table up to 160 KiB, the DFA table up to 4096 entries),
+ *                       read from L2 otherwise; 0 = every lexer table read from L2
+ * graph_launches / graph_captures count the context's batched calls that ran as a graph and the
+ * graphs captured (read by alpacc_get_tuning, ignored by alpacc_set_tuning). */
+typedef struct alpacc_tuning {
+  int32_t graph_mode;
+  uint32_t lex_blocks_per_cu;
+  uint32_t keys_blocks_per_cu;
+  uint32_t search_max_blocks;
+  uint32_t brackets_per_kib;
+  uint32_t search_block_threads;
+  int32_t lexer_tables_in_lds;
+  uint32_t reserved;
+  uint64_t graph_launches;
+  uint64_t graph_captures;
+} alpacc_tuning;
+int alpacc_set_tuning(alpacc_ctx *ctx, const alpacc_tuning *t);
+int alpacc_get_tuning(alpacc_ctx *ctx, alpacc_tuning *t);
+
+/* Device memory a context holds: its tables and its grow-only scratch arenas (the serving loop
+ * admits requests by it). */
+uint64_t alpacc_device_bytes(alpacc_ctx *ctx);
+
 /* Element counts of the last parser call on `ctx`: counts[0] tokens, [1] key positions (T + 2;
  * tile-padded in batched calls), [2] brackets written for the bracket search (those left after
  * neighbouring positions' matching brackets cancel), [3] productions (tree nodes). */
@@ -220,6 +269,7 @@ int alpacc_stream_sync(void *stream);
 void *alpacc_event_create(void);
 void alpacc_event_destroy(void *event);
 int alpacc_event_record(void *event, void *stream);
+int alpacc_event_sync(void *event); /* waits until the recorded work is done */
 /* milliseconds between two recorded events (waits for `end`); negative on error */
 float alpacc_event_elapsed_ms(void *start, void *end);
 
