@@ -1874,6 +1874,9 @@ __global__ void pb_docs(PArgs a) {
 }
 
 constexpr int PX_STAGE = 256;  // element slots per wavefront and round
+
+// LDS slot of position record q of a wavefront (one padding record after every 16)
+__device__ __forceinline__ int px_pad(int q) { return q + (q >> 4); }
 constexpr int PX_TOK = 320;    // token spans per wavefront staged in LDS (32-bit spans)
 
 // dynamic LDS of parse_expand2: stackw (u32, padded to 8 bytes), prodw (uint2), entry infos at a
@@ -1903,7 +1906,10 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
                                                             uint32_t *node, S *span,
                                                             int write_nodes) {
   __shared__ PTup s_w[PK_THREADS / WAVE];
-  __shared__ uint4 s_pos[PK_THREADS / WAVE][WAVE * PK_IPT];   // per position of the wavefront
+  // per position of the wavefront, position q at q + q / 16: the 16-byte records a lane writes
+  // (4 consecutive positions) then start 5 records apart from its neighbour's, so the lanes' stores
+  // spread over all banks (unpadded they were 64 bytes apart: 16 lanes on each bank group)
+  __shared__ uint4 s_pos[PK_THREADS / WAVE][WAVE * PK_IPT + WAVE * PK_IPT / 16];
   __shared__ __align__(16) uint16_t s_mark[PK_THREADS / WAVE][PX_STAGE];    // 1 + position starting at a slot
   // the spans of the tokens the wavefront's terminal productions take (a contiguous rank range)
   constexpr int NTOK = sizeof(S) == 4 ? PX_TOK : 1;
@@ -2014,7 +2020,7 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
     for (int i = 0; i < PK_IPT; ++i) {
       ob[i] = b;
       op[i] = q;
-      pos[lane * PK_IPT + i] = make_uint4((uint32_t)b, (uint32_t)d, (uint32_t)(inf[i].sst + skip[i]), 0u);
+      pos[px_pad(lane * PK_IPT + i)] = make_uint4((uint32_t)b, (uint32_t)d, (uint32_t)(inf[i].sst + skip[i]), 0u);
       b += nbw[i];
       q += inf[i].np;
       d += inf[i].dd;
@@ -2072,7 +2078,7 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
         const bool act = g >= wlo && g < whi;
         int32_t dv = INT32_MAX;
         if (act) {
-          const uint4 p = pos[o - 1];
+          const uint4 p = pos[px_pad(o - 1)];
           const uint32_t sw = s_stackw[p.z + (uint32_t)(g - (int32_t)p.x)];
           dv = (int32_t)p.y + (int32_t)(int16_t)(sw >> 16);
           __builtin_nontemporal_store(dv, depth + g);  // streamed: see the productions below
@@ -2091,7 +2097,7 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
     int32_t q = pre.np + ex.np, sv = pre.ds + ex.ds, r = pre.nt + ex.nt;
 #pragma unroll
     for (int i = 0; i < PK_IPT; ++i) {
-      pos[lane * PK_IPT + i] = make_uint4((uint32_t)q, (uint32_t)sv, (uint32_t)r, inf[i].pst);
+      pos[px_pad(lane * PK_IPT + i)] = make_uint4((uint32_t)q, (uint32_t)sv, (uint32_t)r, inf[i].pst);
       q += inf[i].np;
       sv += inf[i].ds;
       r += inf[i].nt;
@@ -2126,7 +2132,7 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
         v0[j] = 0;
         v1[j] = 0;
         if (g >= wlo && g < whi) {
-          const uint4 p = pos[o - 1];
+          const uint4 p = pos[px_pad(o - 1)];
           const uint2 pw = s_prodw[p.w + (uint32_t)(g - (int32_t)p.x)];
           const int32_t t = subst ? (int32_t)(pw.x >> 16) - 1 : -1;
           svs[j] = (int32_t)p.y + (int32_t)(int16_t)(pw.y & 0xFFFFu);

@@ -368,14 +368,15 @@ def main():
     # Algorithmic bytes per launch of every single-kernel stage: what the kernel must read and
     # write at the least (per unit: byte n, token T, key position P = T + 2, bracket B, node N).
     #   lex_reduce    : n                       reads the document
-    #   lex_emit      : n + 12 T                reads it again, writes (kind, start, end) u32
+    #   lex_emit      : n + 12 T                reads it again, writes (kind, start, end) u32; inside
+    #                   n + 9 T                 alpacc_parse the kinds are bytes
     #   parse_expand  : 2 P + 8 T + 6 B + 16 N  pair entries in; token spans gathered; depth + symbol
     #                                           per bracket; s, node word and span per node out
     #   bracket_match : 6 B                     depth + symbol per bracket
     #   parse_parents : 8 N                     s in, parent out
     cnt = ctx.last_counts() if args.workload == "parse" else {}
     P, B, N_ = cnt.get("positions", 0), cnt.get("brackets", 0), cnt.get("productions", 0)
-    alg = {"lex_reduce": n, "lex_emit": n + 12 * T}
+    alg = {"lex_reduce": n, "lex_emit": n + (9 if args.workload == "parse" else 12) * T}
     if args.workload == "parse":
         alg.update({"parse_expand": 2 * P + 8 * T + 6 * B + 16 * N_, "bracket_match": 6 * B,
                     "parse_parents": 8 * N_})
