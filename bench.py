@@ -96,6 +96,7 @@ def cpu_baseline(bundle, workload, doc, seed, one_thread_mib):
     chunk = 16 << 20
     t_u = _median_time(lambda: fn(doc))
     t_f = _median_time(lambda: fn(doc, chunk=chunk))
+    f_accepts = fn(doc, chunk=chunk) is not None
     small = datagen.json_doc(int(one_thread_mib * (1 << 20)), seed=seed)
     arith_b = generate(os.path.join(ROOT, "alpacc_amd", "grammars", "arithmetic.alp"))
     arith = datagen.arith_doc(1 << 20, seed=seed)
@@ -118,6 +119,8 @@ def cpu_baseline(bundle, workload, doc, seed, one_thread_mib):
                       f"unchunked), {threads} threads, median of 5 runs after one warm-up",
             "cpu_model": _cpu_model(), "nproc": cores,
             "mode_F_GBps": gbs(n, t_f),
+            # the chunk loop lexes the overlap byte twice; a rejection there skips the parse
+            "mode_F_accepts": f_accepts,
             "one_thread": {"GBps": gbs(small.shape[0], t_1),
                            "sample": f"{one_thread_mib:g} MiB JSON document, seed {seed}, mode U"},
             "arithmetic_1MiB": {"all_threads_GBps": gbs(arith.shape[0], t_a),
