@@ -278,17 +278,26 @@ __device__ __forceinline__ void load_classes(const uint8_t *in, bool aligned, co
     }
 #undef LEX_ALIGN_CASE
     classes_of_words(wds, s_cls, cls);
-  } else {
+  } else if (valid > 0) {
+    // a document's first or last lanes: byte loads at clamped (in-bounds) positions, with no
+    // branch between them, so that they are in flight together (a guarded load per byte waits
+    // for each in turn, and the wave tile's whole count tile, and every count tile that looks
+    // back to it, waits for this lane)
+    const uint64_t last = n - 1;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       uint32_t v = 0;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int j = q * 4 + r;
-        if (j < valid) v |= (uint32_t)s_cls[in[base + j]] << (8 * r);
+        const uint32_t c = s_cls[in[min(base + (uint64_t)j, last)]];
+        v |= (j < valid ? c : 0u) << (8 * r);
       }
       cls[q] = v;
     }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) cls[q] = 0u;
   }
 }
 
@@ -298,7 +307,8 @@ __device__ __forceinline__ void load_classes(const uint8_t *in, bool aligned, co
 // TAB: where the tables sit: 2 = both in LDS, 1 = the (endo x class) table in LDS and the
 // composition table in L2, 0 = both in L2 (an (endo x class) table larger than LDS).
 template <int TAB, bool BATCH>
-__global__ void __launch_bounds__(LEX_BLOCK) lex_reduce(LexArgs p) {
+__global__ void __launch_bounds__(LEX_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
+lex_reduce(LexArgs p) {
   extern __shared__ __align__(16) uint8_t smem[];
   const uint32_t e = p.e, C = p.C;
   uint16_t *s_step = reinterpret_cast<uint16_t *>(smem);
