@@ -1411,6 +1411,8 @@ struct PArgs {
   uint32_t ndocs;
   uint32_t *doc_flags;       // [ndocs] bit 0: missing pair, bit 1: bracket mismatch
   uint32_t *chunk_doc;       // [N / 64] document of the first production of every 64-chunk
+  uint32_t *chunk_base;      // [N / 64] first node of that document; bit 31: a later document
+                             // starts inside the chunk
   // batched mode: capacities of the bracket and production scratch, sized before the launch; a
   // batch whose totals exceed them (or 2^31) writes no elements (the host then sizes them again)
   uint64_t bcap, ncap;
@@ -1934,24 +1936,25 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
   // one block per position tile. The batched grid covers a bound of the tile count (the count is
   // known on the device only); a batch that does not fit its element scratch writes nothing.
   const uint64_t tile = blockIdx.x;
+  const uint64_t pbase = tile * PK_TILE + (uint64_t)threadIdx.x * PK_IPT;  // slot
+  // the lane's four entries, loaded before anything the tile's bounds depend on (the document
+  // descriptor, the device token count): the entry array holds every launched tile whole, and
+  // entries past the document's end are not used (pend below)
+  const uint2 packed = *reinterpret_cast<const uint2 *>(a.ent + pbase);
   if (BATCH) {
     if (tile >= *a.ntiles_dev || !batch_elems_fit(a)) return;
   }
   const PosView pv = pos_view<BATCH>(a, tile);
   const int subst = pv.subst;
-  const uint64_t pbase = tile * PK_TILE + (uint64_t)threadIdx.x * PK_IPT;  // slot
   const uint64_t pend = tile * PK_TILE + (pv.P > (uint64_t)pv.p0 ? pv.P - (uint64_t)pv.p0 : 0);
-  uint16_t ids[PK_IPT] = {0, 0, 0, 0};
-  if (pbase + PK_IPT <= pend) {
-    const uint2 packed = *reinterpret_cast<const uint2 *>(a.ent + pbase);
-    ids[0] = (uint16_t)packed.x;
-    ids[1] = (uint16_t)(packed.x >> 16);
-    ids[2] = (uint16_t)packed.y;
-    ids[3] = (uint16_t)(packed.y >> 16);
-  } else {
-    for (int i = 0; i < PK_IPT; ++i)
-      if (pbase + i < pend) ids[i] = a.ent[pbase + i];
+  // batched: the first node of the tile's document and of the next one, for the chunk words
+  uint32_t doc_node0 = 0, next_node0 = 0xFFFFFFFFu;
+  if (BATCH) {
+    doc_node0 = (uint32_t)a.pdocs[pv.doc].node_off;
+    if (pv.doc + 1 < a.ndocs) next_node0 = (uint32_t)a.pdocs[pv.doc + 1].node_off;
   }
+  const uint16_t ids[PK_IPT] = {(uint16_t)packed.x, (uint16_t)(packed.x >> 16), (uint16_t)packed.y,
+                                (uint16_t)(packed.y >> 16)};
   // the tables' copy into LDS overlaps the loads of the entries above
   for (uint32_t i = threadIdx.x; i < a.nstack; i += blockDim.x) s_stackw[i] = a.stackw[i];
   for (uint32_t i = threadIdx.x; i < a.nprod; i += blockDim.x) s_prodw[i] = a.prodw[i];
@@ -2116,6 +2119,7 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
   // productions: s (parents, parser.fut:286-296), node and span (terminal_offsets, :305-332)
   {
     const int32_t wlo = __shfl(op[0], 0, WAVE), whi = __shfl(lp_hi, WAVE - 1, WAVE);
+    const bool root_wave = pv.reset && w == 0;  // the wavefront holding the document's first node
     int32_t carry = 0;
     for (int32_t r0w = wlo & ~(WAVE - 1); r0w < whi; r0w += PX_STAGE) {
       reinterpret_cast<uint2 *>(mark)[lane] = make_uint2(0u, 0u);
@@ -2145,7 +2149,10 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
           const uint4 p = pos[px_pad(o - 1)];
           const uint2 pw = s_prodw[p.w + (uint32_t)(g - (int32_t)p.x)];
           const int32_t t = subst ? (int32_t)(pw.x >> 16) - 1 : -1;
-          svs[j] = (int32_t)p.y + (int32_t)(int16_t)(pw.y & 0xFFFFu);
+          // the document's root (its first production) gets s = -1: every other node has s >= 0 in
+          // a valid tree, so the root still ends every node's search, and the search kernel
+          // tells it by its value, with no document offsets
+          svs[j] = root_wave && g == wlo ? -1 : (int32_t)p.y + (int32_t)(int16_t)(pw.y & 0xFFFFu);
           nds[j] = t >= 0 ? (ALPACC_NODE_TERMINAL | (uint32_t)t) : (pw.x & 0xFFFFu);
           if (write_nodes && t >= 0) {
             const int32_t rk = (int32_t)p.z + (int32_t)(pw.y >> 16);
@@ -2186,7 +2193,11 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
         if (lane == 0) {
           if (cb >= wlo && cb + WAVE <= whi) sl1[cb / WAVE] = m;
           else atomicMin(sl1 + cb / WAVE, m);
-          if (BATCH && cb >= wlo) a.chunk_doc[cb / WAVE] = pv.doc;
+          if (BATCH && cb >= wlo) {
+            a.chunk_doc[cb / WAVE] = pv.doc;
+            a.chunk_base[cb / WAVE] =
+                doc_node0 | ((uint64_t)next_node0 < (uint64_t)cb + WAVE ? 0x80000000u : 0u);
+          }
         }
       }
       wave_lds_sync();
@@ -2280,26 +2291,27 @@ __device__ void parents_run(uint64_t N, const Levels &lv, uint32_t *parent, cons
   for (uint64_t c = blk * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < nch; c += wstride) {
     const uint64_t i = c * WAVE + lane;
     const bool valid = i < N;
-    uint64_t base = 0;
-    if (BATCH) {
-      // the document of node i: the document of the chunk's first node, unless a later document
-      // starts inside the chunk (then forward, per lane)
-      const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.chunk_doc[c]);
-      const uint64_t next = d0 + 1 < a.ndocs ? a.pdocs[d0 + 1].node_off : ~0ull;
-      if (next >= (c + 1) * WAVE) {
-        base = a.pdocs[d0].node_off;
-      } else {
-        uint32_t d = d0;
-        while (d + 1 < a.ndocs && a.pdocs[d + 1].node_off <= i && valid) ++d;
-        base = a.pdocs[d].node_off;
-      }
-    }
+    // batched: the chunk's word (the first node of its first node's document), loaded now and
+    // used after the search, so that its latency hides behind the search's
+    uint32_t cw = 0;
+    if (BATCH) cw = a.chunk_base[c];
     const int32_t v = valid ? lv.L[0][i] : INT32_MAX;
     const SearchRev rv = search_rev(lv, c, lane);
-    const int64_t j =
-        prev_le_wave(lv, c, lane, v, rv, valid && i > base, lift_left(v, lane, v));
-    if (valid)
-      __builtin_nontemporal_store(i == base ? 0u : (uint32_t)((uint64_t)j - base), parent + i);
+    // a document's root has s = -1 (every other node s >= 0): it searches nothing and is its own
+    // parent
+    const int64_t j = prev_le_wave(lv, c, lane, v, rv, valid && v >= 0, lift_left(v, lane, v));
+    if (valid) {
+      uint64_t base = 0;
+      if (BATCH) {
+        base = cw & 0x7FFFFFFFu;
+        if (cw >> 31) {  // a later document starts inside the chunk: forward from the chunk's
+          uint32_t d = a.chunk_doc[c];
+          while (d + 1 < a.ndocs && a.pdocs[d + 1].node_off <= i) ++d;
+          base = a.pdocs[d].node_off;
+        }
+      }
+      __builtin_nontemporal_store(v < 0 ? 0u : (uint32_t)((uint64_t)j - base), parent + i);
+    }
   }
 }
 
@@ -2539,6 +2551,7 @@ enum BufId {
   B_PTILE_DOC,
   B_RESULTS,
   B_CHUNK_DOC,
+  B_CHUNK_BASE,
 };
 
 template <typename T>
@@ -3303,12 +3316,13 @@ int plan_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offse
   pl.e.sx = (int32_t *)ctx->buf(B_SX, (ncap + 16) * sizeof(int32_t));
   pl.e.sl1 = (int32_t *)ctx->buf(B_SL1, (npc + 16) * sizeof(int32_t));
   uint32_t *chunk_doc = (uint32_t *)ctx->buf(B_CHUNK_DOC, (npc + 2) * 4);
+  uint32_t *chunk_base = (uint32_t *)ctx->buf(B_CHUNK_BASE, (npc + 2) * 4);
   const size_t docs_bytes = ((ndocs + 1) * sizeof(LexDoc) + 255) & ~(size_t)255;
   const size_t res_bytes = ((ndocs + 1) * sizeof(alpacc_doc_result) + 255) & ~(size_t)255;
   uint8_t *stg = ctx->stage(docs_bytes + res_bytes + 256, st);
   if (!kinds || !starts || !ends || !doc_res || !pdocs || !doc_flags || !ptile_doc || !ent ||
       !tiles || !pre || !pl.d_results || !pl.btot || !pl.e.depth || !pl.e.br || !pl.e.dl1 ||
-      !pl.e.sx || !pl.e.sl1 || !chunk_doc || !stg) {
+      !pl.e.sx || !pl.e.sl1 || !chunk_doc || !chunk_base || !stg) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
@@ -3334,6 +3348,7 @@ int plan_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offse
   a.ndocs = ndocs;
   a.doc_flags = doc_flags;
   a.chunk_doc = chunk_doc;
+  a.chunk_base = chunk_base;
   a.bcap = bcap;
   a.ncap = ncap;
   pl.tiles_max = tiles_max;
