@@ -165,6 +165,7 @@ struct LexArgs {
   // at 0 in every document, and document d's tokens go to slots [tok_base, tok_base + len + 1).
   const struct LexDoc *docs;  // [ndocs]
   const uint32_t *tile_doc;   // [num_tiles] document of every wave tile
+  const uint4 *tile_rec;      // [num_tiles] the document's byte offset (x, y), length, tile in it
   uint32_t ndocs;
   uint32_t *doc_res;          // [2*ndocs] accept flag, token count
 };
@@ -199,13 +200,15 @@ __device__ __forceinline__ TileView tile_view(const LexArgs &p, uint32_t tile) {
     v.ltile = tile;
     v.aligned = p.aligned != 0;
   } else {
+    // the input loads wait on one record (offset, length, tile in the document); the document's
+    // index and token base are only needed later, when the tokens are written
+    const uint4 r = p.tile_rec[tile];
     const uint32_t d = p.tile_doc[tile];
-    const LexDoc doc = p.docs[d];
-    v.ptr = p.in + doc.offset;
-    v.n = doc.len;
-    v.tok_base = doc.tok_base;
+    v.ptr = p.in + (((uint64_t)r.y << 32) | r.x);
+    v.n = r.z;
+    v.tok_base = p.docs[d].tok_base;
     v.doc = d;
-    v.ltile = tile - doc.tile_base;
+    v.ltile = r.w;
     v.aligned = ((uintptr_t)v.ptr & 15) == 0;
   }
   return v;
@@ -530,11 +533,14 @@ __global__ void __launch_bounds__(LEX_SCAN_BLOCK) lex_scan_docs(LexArgs p) {
   }
 }
 
-// Batched mode: the document of every wave tile.
+// Batched mode: the document of every wave tile, and the record its input loads start from.
 __global__ void lex_tile_docs(LexArgs p) {
   const LexDoc doc = p.docs[blockIdx.x];
-  for (uint32_t i = threadIdx.x; i < doc.ntiles; i += blockDim.x)
+  for (uint32_t i = threadIdx.x; i < doc.ntiles; i += blockDim.x) {
     const_cast<uint32_t *>(p.tile_doc)[doc.tile_base + i] = blockIdx.x;
+    const_cast<uint4 *>(p.tile_rec)[doc.tile_base + i] =
+        make_uint4((uint32_t)doc.offset, (uint32_t)(doc.offset >> 32), (uint32_t)doc.len, i);
+  }
 }
 
 // Exclusive (token count, last token start) of a count tile: decoupled look-back; the two
@@ -2551,6 +2557,7 @@ enum BufId {
   B_PTILE_DOC,
   B_RESULTS,
   B_CHUNK_DOC,
+  B_TILE_REC,
   B_CHUNK_BASE,
 };
 
@@ -2821,6 +2828,7 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   a.chunk_info = chunk_info;
   a.docs = nullptr;
   a.tile_doc = nullptr;
+  a.tile_rec = nullptr;
   a.ndocs = 0;
   a.doc_res = nullptr;
   const uint64_t wpb = LEX_BLOCK / WAVE;
@@ -2923,7 +2931,8 @@ int plan_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets
   a.tile_state = (uint8_t *)ctx->buf(B_TILE_STATE, tiles);
   a.st_count = (uint64_t *)ctx->buf(B_ST_COUNT, ctiles * sizeof(uint64_t));
   a.tile_doc = (uint32_t *)ctx->buf(B_TILE_DOC, tiles * sizeof(uint32_t));
-  if (!a.lane_excl || !a.tile_agg || !a.tile_state || !a.st_count || !a.tile_doc) {
+  a.tile_rec = (uint4 *)ctx->buf(B_TILE_REC, tiles * sizeof(uint4));
+  if (!a.lane_excl || !a.tile_agg || !a.tile_state || !a.st_count || !a.tile_doc || !a.tile_rec) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
