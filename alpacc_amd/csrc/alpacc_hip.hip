@@ -166,6 +166,7 @@ struct LexArgs {
   const struct LexDoc *docs;  // [ndocs]
   const uint32_t *tile_doc;   // [num_tiles] document of every wave tile
   const uint4 *tile_rec;      // [num_tiles] the document's byte offset (x, y), length, tile in it
+  const uint2 *chunks;        // [nchunks] first wave tile, tile count of every scan chunk
   uint32_t ndocs;
   uint32_t *doc_res;          // [2*ndocs] accept flag, token count
 };
@@ -176,6 +177,8 @@ struct LexDoc {
   uint64_t tok_base;  // first token slot
   uint32_t tile_base; // first wave tile
   uint32_t ntiles;
+  uint32_t chunk_base;  // first scan chunk (LEX_SCAN_BLOCK wave tiles of this document)
+  uint32_t nchunks;
 };
 
 // The bytes a wave tile covers: document base pointer and length, the tile's index inside its
@@ -428,14 +431,15 @@ __global__ void __launch_bounds__(LEX_SCAN_BLOCK) lex_scan_tiles(LexArgs p) {
   if (t < p.num_tiles) p.tile_excl[t] = s[i];
 }
 
-// K2b: one block, exclusive composition of the block totals in place (chunks with a carry).
-__global__ void __launch_bounds__(LEX_SCAN_BLOCK) lex_scan_blocks(LexArgs p, uint32_t nblocks) {
+// Exclusive composition of v[0 .. n) in place by one workgroup of LEX_SCAN_BLOCK threads
+// (Blelloch in LDS, chunks with a carry).
+__device__ void scan_excl_inplace(const LexArgs &p, uint16_t *v, uint32_t n) {
   __shared__ uint16_t s[LEX_SCAN_BLOCK];
   __shared__ uint16_t carry_s;
   const uint32_t i = threadIdx.x;
   if (i == 0) carry_s = 0;
-  for (uint32_t off = 0; off < nblocks; off += LEX_SCAN_BLOCK) {
-    s[i] = off + i < nblocks ? p.block_agg[off + i] : 0;
+  for (uint32_t off = 0; off < n; off += LEX_SCAN_BLOCK) {
+    s[i] = off + i < n ? v[off + i] : 0;
     __syncthreads();
     for (uint32_t d = 1; d < LEX_SCAN_BLOCK; d <<= 1) {
       uint32_t nv = 0;
@@ -466,11 +470,16 @@ __global__ void __launch_bounds__(LEX_SCAN_BLOCK) lex_scan_blocks(LexArgs p, uin
       __syncthreads();
     }
     const uint32_t carry = carry_s;
-    if (off + i < nblocks) p.block_agg[off + i] = (uint16_t)comp_lookup(p.endo_comp, p.e, carry, s[i]);
+    if (off + i < n) v[off + i] = (uint16_t)comp_lookup(p.endo_comp, p.e, carry, s[i]);
     __syncthreads();
     if (i == LEX_SCAN_BLOCK - 1) carry_s = (uint16_t)comp_lookup(p.endo_comp, p.e, carry, chunk_total);
     __syncthreads();
   }
+}
+
+// K2b: one block, exclusive composition of the block totals in place.
+__global__ void __launch_bounds__(LEX_SCAN_BLOCK) lex_scan_blocks(LexArgs p, uint32_t nblocks) {
+  scan_excl_inplace(p, p.block_agg, nblocks);
 }
 
 // K2c: DFA state at the start of every wave tile.
@@ -481,55 +490,59 @@ __global__ void lex_tile_states(LexArgs p) {
   p.tile_state[t] = p.endo_apply[pre * p.m1 + p.init];
 }
 
-// K2 of the batched mode: one block per document; exclusive composition of the document's wave
-// tiles (Blelloch in LDS, chunks with a carry), then the DFA state at the start of every tile.
-__global__ void __launch_bounds__(LEX_SCAN_BLOCK) lex_scan_docs(LexArgs p) {
+// K2 of the batched mode, spread over many workgroups however long a document is (one workgroup per
+// document took 0.6 ms for a 256 MiB one): (a) every chunk of up to LEX_SCAN_BLOCK wave tiles of
+// one document, exclusive composition inside the chunk and the chunk's total; (b) per document,
+// exclusive composition of its chunk totals in place; (c) the DFA state at the start of every tile.
+__global__ void __launch_bounds__(LEX_SCAN_BLOCK) lex_scan_chunks(LexArgs p) {
   __shared__ uint16_t s[LEX_SCAN_BLOCK];
-  __shared__ uint16_t carry_s;
-  const LexDoc doc = p.docs[blockIdx.x];
+  const uint2 ch = p.chunks[blockIdx.x];
   const uint32_t i = threadIdx.x;
-  if (i == 0) carry_s = 0;
-  for (uint32_t off = 0; off < doc.ntiles; off += LEX_SCAN_BLOCK) {
-    const uint32_t t = doc.tile_base + off + i;
-    const bool in = off + i < doc.ntiles;
-    s[i] = in ? p.tile_agg[t] : 0;
+  const bool in = i < ch.y;
+  s[i] = in ? p.tile_agg[ch.x + i] : 0;  // 0: the identity
+  __syncthreads();
+  for (uint32_t d = 1; d < LEX_SCAN_BLOCK; d <<= 1) {
+    uint32_t nv = 0;
+    const bool act = ((i + 1) & (2 * d - 1)) == 0;
+    if (act) nv = comp_lookup(p.endo_comp, p.e, s[i - d], s[i]);
     __syncthreads();
-    for (uint32_t d = 1; d < LEX_SCAN_BLOCK; d <<= 1) {
-      uint32_t nv = 0;
-      const bool act = ((i + 1) & (2 * d - 1)) == 0;
-      if (act) nv = comp_lookup(p.endo_comp, p.e, s[i - d], s[i]);
-      __syncthreads();
-      if (act) s[i] = (uint16_t)nv;
-      __syncthreads();
-    }
-    uint32_t chunk_total = 0;
-    if (i == LEX_SCAN_BLOCK - 1) {
-      chunk_total = s[i];
-      s[i] = 0;
-    }
+    if (act) s[i] = (uint16_t)nv;
     __syncthreads();
-    for (uint32_t d = LEX_SCAN_BLOCK / 2; d >= 1; d >>= 1) {
-      const bool right = ((i + 1) & (2 * d - 1)) == 0;
-      uint32_t l = 0, r = 0;
-      if (right) {
-        l = s[i - d];
-        r = s[i];
-      }
-      __syncthreads();
-      if (right) {
-        s[i - d] = (uint16_t)r;
-        s[i] = (uint16_t)comp_lookup(p.endo_comp, p.e, r, l);
-      }
-      __syncthreads();
-    }
-    const uint32_t carry = carry_s;
-    if (in) {
-      const uint32_t pre = comp_lookup(p.endo_comp, p.e, carry, s[i]);
-      p.tile_state[t] = p.endo_apply[pre * p.m1 + p.init];
+  }
+  if (i == LEX_SCAN_BLOCK - 1) {
+    p.block_agg[blockIdx.x] = s[i];
+    s[i] = 0;
+  }
+  __syncthreads();
+  for (uint32_t d = LEX_SCAN_BLOCK / 2; d >= 1; d >>= 1) {
+    const bool right = ((i + 1) & (2 * d - 1)) == 0;
+    uint32_t l = 0, r = 0;
+    if (right) {
+      l = s[i - d];
+      r = s[i];
     }
     __syncthreads();
-    if (i == LEX_SCAN_BLOCK - 1) carry_s = (uint16_t)comp_lookup(p.endo_comp, p.e, carry, chunk_total);
+    if (right) {
+      s[i - d] = (uint16_t)r;
+      s[i] = (uint16_t)comp_lookup(p.endo_comp, p.e, r, l);
+    }
     __syncthreads();
+  }
+  if (in) p.tile_excl[ch.x + i] = s[i];
+}
+
+__global__ void __launch_bounds__(LEX_SCAN_BLOCK) lex_scan_doc_chunks(LexArgs p) {
+  const LexDoc doc = p.docs[blockIdx.x];
+  scan_excl_inplace(p, p.block_agg + doc.chunk_base, doc.nchunks);
+}
+
+__global__ void __launch_bounds__(LEX_SCAN_BLOCK) lex_chunk_states(LexArgs p) {
+  const uint2 ch = p.chunks[blockIdx.x];
+  const uint32_t i = threadIdx.x;
+  if (i < ch.y) {
+    const uint32_t pre =
+        comp_lookup(p.endo_comp, p.e, p.block_agg[blockIdx.x], p.tile_excl[ch.x + i]);
+    p.tile_state[ch.x + i] = p.endo_apply[pre * p.m1 + p.init];
   }
 }
 
@@ -541,6 +554,10 @@ __global__ void lex_tile_docs(LexArgs p) {
     const_cast<uint4 *>(p.tile_rec)[doc.tile_base + i] =
         make_uint4((uint32_t)doc.offset, (uint32_t)(doc.offset >> 32), (uint32_t)doc.len, i);
   }
+  // and the document's scan chunks (first wave tile, tile count)
+  for (uint32_t c = threadIdx.x; c < doc.nchunks; c += blockDim.x)
+    const_cast<uint2 *>(p.chunks)[doc.chunk_base + c] =
+        make_uint2(doc.tile_base + c * LEX_SCAN_BLOCK, min((uint32_t)LEX_SCAN_BLOCK, doc.ntiles - c * LEX_SCAN_BLOCK));
 }
 
 // Exclusive (token count, last token start) of a count tile: decoupled look-back; the two
@@ -2558,6 +2575,7 @@ enum BufId {
   B_RESULTS,
   B_CHUNK_DOC,
   B_TILE_REC,
+  B_CHUNKS,
   B_CHUNK_BASE,
 };
 
@@ -2829,6 +2847,7 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   a.docs = nullptr;
   a.tile_doc = nullptr;
   a.tile_rec = nullptr;
+  a.chunks = nullptr;
   a.ndocs = 0;
   a.doc_res = nullptr;
   const uint64_t wpb = LEX_BLOCK / WAVE;
@@ -2870,7 +2889,7 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
 // captured into a HIP graph and replayed.
 struct LexBatchPlan {
   uint32_t ndocs;
-  uint64_t tiles, ctiles;
+  uint64_t tiles, ctiles, nchunks;
   unsigned lex_blocks;
   LexArgs a;
   LexDoc *d_docs;  // device copy of the descriptors
@@ -2892,15 +2911,18 @@ int plan_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets
     ctx->err = "bundle has no lexer";
     return ALPACC_ERR_ARG;
   }
-  uint64_t tiles = 0, toks = 0;
+  uint64_t tiles = 0, toks = 0, chunks = 0;
   for (uint32_t d = 0; d < ndocs; ++d) {
     if (lengths[d] >= (1ull << 31) - 2) {
       ctx->err = "documents must be shorter than 2^31 - 2 bytes";
       return ALPACC_ERR_ARG;
     }
     const uint64_t nt = (lengths[d] + LEX_TILE - 1) / LEX_TILE;
-    h_docs[d] = LexDoc{offsets[d], lengths[d], toks, (uint32_t)tiles, (uint32_t)nt};
+    const uint64_t nch = (nt + LEX_SCAN_BLOCK - 1) / LEX_SCAN_BLOCK;
+    h_docs[d] = LexDoc{offsets[d], lengths[d], toks, (uint32_t)tiles, (uint32_t)nt,
+                       (uint32_t)chunks, (uint32_t)nch};
     tiles += nt;
+    chunks += nch;
     toks += lengths[d] + 1;
     if (tiles >= (1ull << 31) / LEX_TILE * 2) {
       ctx->err = "batch too large";
@@ -2932,7 +2954,12 @@ int plan_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets
   a.st_count = (uint64_t *)ctx->buf(B_ST_COUNT, ctiles * sizeof(uint64_t));
   a.tile_doc = (uint32_t *)ctx->buf(B_TILE_DOC, tiles * sizeof(uint32_t));
   a.tile_rec = (uint4 *)ctx->buf(B_TILE_REC, tiles * sizeof(uint4));
-  if (!a.lane_excl || !a.tile_agg || !a.tile_state || !a.st_count || !a.tile_doc || !a.tile_rec) {
+  a.chunks = (uint2 *)ctx->buf(B_CHUNKS, chunks * sizeof(uint2));
+  a.tile_excl = (uint16_t *)ctx->buf(B_TILE_EXCL, tiles * sizeof(uint16_t));
+  a.block_agg = (uint16_t *)ctx->buf(B_BLOCK_AGG, chunks * sizeof(uint16_t));
+  pl.nchunks = chunks;
+  if (!a.lane_excl || !a.tile_agg || !a.tile_state || !a.st_count || !a.tile_doc || !a.tile_rec ||
+      !a.chunks || !a.tile_excl || !a.block_agg) {
     ctx->err = "scratch allocation failed";
     return ALPACC_ERR_HIP;
   }
@@ -2979,12 +3006,14 @@ void enqueue_lex_batch(alpacc_ctx *ctx, const LexBatchPlan &pl, hipStream_t st) 
   launch_lex_reduce<true>(pl.lex_tab, pl.lex_blocks, pl.lex_lds, a, st);
   ctx->prof_end(S_LEX_REDUCE, st);
   ctx->prof_begin(S_LEX_SCAN, st);
-  lex_scan_docs<<<pl.ndocs, LEX_SCAN_BLOCK, 0, st>>>(a);
+  lex_scan_chunks<<<(unsigned)pl.nchunks, LEX_SCAN_BLOCK, 0, st>>>(a);
+  lex_scan_doc_chunks<<<pl.ndocs, LEX_SCAN_BLOCK, 0, st>>>(a);
+  lex_chunk_states<<<(unsigned)pl.nchunks, LEX_SCAN_BLOCK, 0, st>>>(a);
   ctx->prof_end(S_LEX_SCAN, st);
   ctx->prof_begin(S_LEX_EMIT, st);
   launch_lex_emit<false, true>(pl.dfa_lds != 0, (unsigned)pl.ctiles, a, st);
   ctx->prof_end(S_LEX_EMIT, st);
-  ctx->launches += 5;
+  ctx->launches += 7;
 }
 
 // The batched lexer queued on `st` (descriptors through the context's pinned staging);
