@@ -1959,14 +1959,14 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
   // one block per position tile. The batched grid covers a bound of the tile count (the count is
   // known on the device only); a batch that does not fit its element scratch writes nothing.
   const uint64_t tile = blockIdx.x;
-  const uint64_t pbase = tile * PK_TILE + (uint64_t)threadIdx.x * PK_IPT;  // slot
-  // the lane's four entries, loaded before anything the tile's bounds depend on (the document
-  // descriptor, the device token count): the entry array holds every launched tile whole, and
-  // entries past the document's end are not used (pend below)
-  const uint2 packed = *reinterpret_cast<const uint2 *>(a.ent + pbase);
   if (BATCH) {
     if (tile >= *a.ntiles_dev || !batch_elems_fit(a)) return;
   }
+  const uint64_t pbase = tile * PK_TILE + (uint64_t)threadIdx.x * PK_IPT;  // slot
+  // the lane's four entries, loaded before anything else the tile's bounds depend on (the
+  // document descriptor, the device token count): the entry array holds every launched tile
+  // whole, and entries past the document's end are not used (pend below)
+  const uint2 packed = *reinterpret_cast<const uint2 *>(a.ent + pbase);
   const PosView pv = pos_view<BATCH>(a, tile);
   const int subst = pv.subst;
   const uint64_t pend = tile * PK_TILE + (pv.P > (uint64_t)pv.p0 ? pv.P - (uint64_t)pv.p0 : 0);
