@@ -1436,6 +1436,7 @@ struct PArgs {
   uint32_t *chunk_doc;       // [N / 64] document of the first production of every 64-chunk
   uint32_t *chunk_base;      // [N / 64] first node of that document; bit 31: a later document
                              // starts inside the chunk
+  uint64_t expand_tile0;     // the striding parse_expand2's first tile
   // batched mode: capacities of the bracket and production scratch, sized before the launch; a
   // batch whose totals exceed them (or 2^31) writes no elements (the host then sizes them again)
   uint64_t bcap, ncap;
@@ -1933,8 +1934,8 @@ constexpr size_t PX_TABLE_LDS_MAX = 32 << 10;  // stackw + prodw + entry infos, 
 // INT32_MAX.
 // Five workgroups of 4 waves per CU (84 VGPRs): without the hint the compiler settles at 106
 // VGPRs (4 waves per SIMD) for the batched and device-sized forms.
-template <typename S, bool BATCH>
-__global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu(5)))
+template <typename S, bool BATCH, bool STRIDE = false>
+__global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu(STRIDE ? 4 : 5)))
 parse_expand2(PArgs a, const S *starts, const S *ends,
                                                             int32_t *depth, uint16_t *br,
                                                             int32_t *dl1, int32_t *sx, int32_t *sl1,
@@ -1956,10 +1957,15 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
   uint32_t *s_can = reinterpret_cast<uint32_t *>(s_einfo + a.E);
   const int lane = threadIdx.x & (WAVE - 1);
   const int w = threadIdx.x / WAVE;
-  // one block per position tile. The batched grid covers a bound of the tile count (the count is
-  // known on the device only); a batch that does not fit its element scratch writes nothing.
-  const uint64_t tile = blockIdx.x;
-  if (BATCH) {
+  // One block per position tile. The batched tile count is known on the device only: the grid
+  // covers an estimate of it, and blocks past the count return at once; STRIDE: the tiles from
+  // a.expand_tile0 on that the estimate missed, by a grid of a few blocks per CU striding over
+  // them (none in the rule: the estimate grows to what calls needed). A batch that does not fit
+  // its element scratch writes nothing.
+  const uint64_t t_first = STRIDE ? a.expand_tile0 + blockIdx.x : (uint64_t)blockIdx.x;
+  const uint64_t t_end = STRIDE ? (batch_elems_fit(a) ? *a.ntiles_dev : 0) : t_first + 1;
+  for (uint64_t tile = t_first; tile < t_end; tile += STRIDE ? gridDim.x : 1) {
+  if (BATCH && !STRIDE) {
     if (tile >= *a.ntiles_dev || !batch_elems_fit(a)) return;
   }
   const uint64_t pbase = tile * PK_TILE + (uint64_t)threadIdx.x * PK_IPT;  // slot
@@ -1978,12 +1984,14 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
   }
   const uint16_t ids[PK_IPT] = {(uint16_t)packed.x, (uint16_t)(packed.x >> 16), (uint16_t)packed.y,
                                 (uint16_t)(packed.y >> 16)};
-  // the tables' copy into LDS overlaps the loads of the entries above
-  for (uint32_t i = threadIdx.x; i < a.nstack; i += blockDim.x) s_stackw[i] = a.stackw[i];
-  for (uint32_t i = threadIdx.x; i < a.nprod; i += blockDim.x) s_prodw[i] = a.prodw[i];
-  for (uint32_t i = threadIdx.x; i < a.E; i += blockDim.x) {
-    s_einfo[i] = a.einfo[i];
-    s_can[i] = a.ecan[i];
+  // the tables' copy into LDS (once per block) overlaps the loads of the entries above
+  if (tile == t_first) {
+    for (uint32_t i = threadIdx.x; i < a.nstack; i += blockDim.x) s_stackw[i] = a.stackw[i];
+    for (uint32_t i = threadIdx.x; i < a.nprod; i += blockDim.x) s_prodw[i] = a.prodw[i];
+    for (uint32_t i = threadIdx.x; i < a.E; i += blockDim.x) {
+      s_einfo[i] = a.einfo[i];
+      s_can[i] = a.ecan[i];
+    }
   }
   const PTup pre = a.pre[tile];
   __syncthreads();
@@ -2226,6 +2234,8 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
       wave_lds_sync();
     }
   }
+  if (STRIDE) __syncthreads();  // the block's waves start the next tile together
+  }  // tiles
 }
 
 // chunk minima of both element streams start at INT32_MAX (chunks shared by two wavefronts take
@@ -2462,6 +2472,8 @@ struct alpacc_ctx {
   uint64_t graph_launches = 0, graph_captures = 0;
   // brackets per input byte the batched scratch is sized for (grows with what calls need)
   double b_per_byte = 0.25;
+  // key positions per input byte the batched expansion's grid is sized for (grows likewise)
+  double p_per_byte = 0.25;
   // launch shapes (alpacc_set_tuning): persistent lex_reduce / parse_keys2 blocks per CU, and the
   // largest grid of the search kernels
   uint32_t lex_bpc = 2, keys_bpc = 2, search_cap = 1u << 16, search_bs = 256;
@@ -3298,6 +3310,8 @@ struct ParseBatchPlan {
   LexBatchPlan lx;
   PArgs a;
   uint64_t tiles_max;
+  uint64_t tiles_est;      // parse_expand2's grid
+  unsigned stride_blocks;  // the striding pass's grid (0: none needed)
   SegTup *btot;
   ElemBufs e;
   uint64_t nbc, npc;  // 64-element chunks of the bracket / production capacities
@@ -3390,6 +3404,13 @@ int plan_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offse
   a.bcap = bcap;
   a.ncap = ncap;
   pl.tiles_max = tiles_max;
+  // parse_expand2's grid: the tiles of an estimate of the key positions (p_per_byte of the input,
+  // grown to what calls needed; the bound from the input bytes is several times the count), and
+  // a striding pass over any tiles past it
+  pl.tiles_est = std::min<uint64_t>(
+      tiles_max, (uint64_t)(ctx->p_per_byte * (double)nbytes) / PK_TILE + ndocs + 1);
+  pl.stride_blocks =
+      pl.tiles_est < tiles_max ? (unsigned)std::max(1, ctx->cus) * 4 : 0u;
   pl.nbc = nbc;
   pl.npc = npc;
   pl.parent = parent;
@@ -3424,11 +3445,20 @@ void enqueue_parse_batch(alpacc_ctx *ctx, const ParseBatchPlan &pl, hipStream_t 
   ctx->launches += 2;
   if (pl.tiles_max > 0) {
     ctx->prof_begin(S_PARSE_EXPAND, st);
-    parse_expand2<uint32_t, true><<<(unsigned)pl.tiles_max, PK_THREADS, pl.xlds, st>>>(
+    parse_expand2<uint32_t, true><<<(unsigned)std::max<uint64_t>(1, pl.tiles_est), PK_THREADS,
+                                     pl.xlds, st>>>(
         a, pl.starts, pl.ends, pl.e.depth, pl.e.br, pl.e.dl1, pl.e.sx, pl.e.sl1, pl.node, pl.span,
         1);
-    ctx->prof_end(S_PARSE_EXPAND, st);
     ctx->launches++;
+    if (pl.stride_blocks) {
+      PArgs as = a;
+      as.expand_tile0 = pl.tiles_est;
+      parse_expand2<uint32_t, true, true><<<pl.stride_blocks, PK_THREADS, pl.xlds, st>>>(
+          as, pl.starts, pl.ends, pl.e.depth, pl.e.br, pl.e.dl1, pl.e.sx, pl.e.sl1, pl.node,
+          pl.span, 1);
+      ctx->launches++;
+    }
+    ctx->prof_end(S_PARSE_EXPAND, st);
   }
   (void)fill_levels(ctx, pl.lvb, pl.lvs, st);
   if (a.bcap > 0) {
@@ -3553,6 +3583,9 @@ int run_parse_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offset
       ctx->err = "output capacity too small";
       return ALPACC_ERR_CAPACITY;
     }
+    if (ntiles > pl.tiles_est && nbytes)  // the striding pass ran: widen the estimate
+      ctx->p_per_byte = std::min(
+          4.0, std::max(ctx->p_per_byte, 1.25 * (double)(ntiles * PK_TILE) / (double)nbytes));
     if (B > bcap) {  // the first pass wrote nothing: size the brackets for this batch
       bcap = B + B / 8 + 4096;
       if (nbytes) ctx->b_per_byte = std::max(ctx->b_per_byte, (double)bcap / (double)nbytes);

@@ -112,3 +112,30 @@ def test_batch_capacity_retry(ctxs):
     orc = coracle.Oracle(b)
     for i, d in enumerate(docs):
         assert _tree(got[i]) == _tree(orc.parse(d))
+
+
+def test_dense_tokens_past_the_tile_estimate():
+    """The batched expansion's grid covers an estimate of the key positions (a quarter of the input
+    bytes at first); documents of one-byte tokens have four times as many, so a fresh context's
+    first call expands most tiles in the striding pass. Both that call and the next (estimate
+    grown) must give every document what it gets alone."""
+    from alpacc_amd.gen.tables import generate
+    from alpacc_amd.runtime import Context
+    from oracle.c import coracle
+    b = generate(os.path.join(GRAMMARS, "sexp.alp"))
+    orc = coracle.Oracle(b)
+    docs = []
+    for i in range(20):
+        k = 20_000 + 997 * i
+        d = b"(" * k + b"a" + b")" * k
+        if i == 7:
+            d = d[:-1]  # unbalanced: rejected
+        docs.append(d)
+    ctx = Context(b)
+    try:
+        for call in range(2):
+            got = ctx.parse_int_batch(docs)
+            for i, d in enumerate(docs):
+                assert _tree(got[i]) == _tree(orc.parse(d)), (call, i, len(d))
+    finally:
+        ctx.close()
