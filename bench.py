@@ -44,6 +44,8 @@ def parse_args():
     ap.add_argument("--batch-total-mib", type=int, default=2048)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving comparison")
+    ap.add_argument("--no-lex", action="store_true", help="skip the 512 MiB lexer-only run")
+    ap.add_argument("--lex-mib", type=int, default=512, help="document size of the lexer-only run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run device-to-device copies of this size (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -127,6 +129,59 @@ def cpu_baseline(bundle, workload, doc, seed, one_thread_mib):
                                 "one_thread_GBps": gbs(arith.shape[0], t_a1),
                                 "ms_all_threads": round(t_a * 1e3, 2),
                                 "workload": "grammars/arithmetic.alp, 1 MiB, seed 42, lex+parse"}}
+
+
+def lexer_only(args, lib):
+    """The GPU lexer alone on one 512 MiB JSON document (the lexer-only bundle: u32 kind, start and
+    end per token), as the plain method's lexer configuration states it."""
+    import numpy as np
+    from alpacc_amd import datagen
+    from alpacc_amd.gen.tables import generate
+    from alpacc_amd.runtime import Context, DeviceBuffer
+    bundle = generate(os.path.join(ROOT, "alpacc_amd", "grammars", "json.alp"), mode="lexer")
+    n = args.lex_mib << 20
+    log(f"generating {args.lex_mib} MiB JSON for the lexer (seed {args.seed})")
+    doc = datagen.json_doc(n, seed=args.seed)
+    ctx = Context(bundle)
+    d_in = DeviceBuffer(n, lib)
+    d_in.upload(doc)
+    del doc
+    outs = [DeviceBuffer(4 * n, lib) for _ in range(3)]
+
+    def step():
+        return ctx.lex_device(d_in.ptr, n, outs[0].ptr, outs[1].ptr, outs[2].ptr, n)
+
+    for _ in range(2):
+        rc, T = step()
+    if rc != 1:
+        raise SystemExit(f"lexer benchmark document rejected (rc={rc})")
+    ctx.profile(True)
+    lib.alpacc_stream_sync(None)
+    t0 = time.perf_counter()
+    steps = 5
+    for _ in range(steps):
+        rc, cnt = step()
+        assert rc == 1 and cnt == T
+    lib.alpacc_stream_sync(None)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    prof = ctx.read_profile()
+    ctx.profile(False)
+    stage = {k: v[0] / v[1] for k, v in prof.items() if v[1]}
+    emit_alg = n + 12 * T  # the document in, u32 kind, start and end per token out
+    emit_tbs = emit_alg / (stage["lex_emit"] / 1e3) / 1e12
+    for o in outs + [d_in]:
+        o.free()
+    ctx.close()
+    return {"workload": f"json lex only, {args.lex_mib} MiB document (seed {args.seed}), "
+                        "lexer-only bundle, u32 kind/start/end per token",
+            "GBps": round(n / (ms / 1e3) / 1e9, 3), "ms_per_step": round(ms, 4), "steps": steps,
+            "tokens": int(T), "tokens_per_s": round(T / (ms / 1e3), 1),
+            "stage_ms": {k: round(v, 4) for k, v in stage.items()},
+            "lex_emit_roofline": {"algorithmic_bytes": int(emit_alg), "achieved_TBps": round(emit_tbs, 3),
+                                  "frac": round(emit_tbs * 1e3 / HBM_PEAK_GBS, 4)},
+            "pipeline_roofline": {"algorithmic_bytes": int(emit_alg),
+                                  "device_ms": round(sum(stage.values()), 4),
+                                  "frac": round(emit_alg / (sum(stage.values()) / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
 def batched(args, lib):
@@ -436,11 +491,13 @@ def main():
             cpu = cpu_baseline(bundle, args.workload, doc, args.seed, args.cpu_sample_mib)
         except Exception as exc:  # the baseline is reported, never required
             log(f"cpu baseline unavailable: {exc}")
-    if not args.no_batch or not args.no_serve:
+    if not args.no_batch or not args.no_serve or not args.no_lex:
         for o in outs:
             o.free()
         d_in.free()
         ctx.close()
+    if not args.no_lex and args.workload == "parse":
+        extra["lexer_only"] = lexer_only(args, lib)
     if not args.no_batch:
         extra["batched"] = batched(args, lib)
     if not args.no_serve:
