@@ -251,10 +251,12 @@ __device__ __forceinline__ void classes_of_words(const uint32_t (&wds)[8], const
   }
 }
 
-// The byte classes of in[base .. base + valid) (n: the document length). A document that starts
-// off a 16-byte boundary is read through the aligned 16-byte words around each lane's 32 bytes
-// (the offset is the same for all lanes of a document), as long as those words lie inside the
-// document; the rest reads bytes.
+// The byte classes of in[base .. base + valid) (n: the document length), class 0 past valid. A
+// document that starts off a 16-byte boundary is read through the aligned 16-byte words around
+// each lane's 32 bytes (the offset is the same for all lanes of a document). A document's first
+// and last lanes read the same words, the last one clamped to the word holding the document's
+// last byte: an aligned 16-byte word that holds a byte of the document lies inside one mapped
+// page, so these loads cannot fault, and the bytes they read outside the document are masked.
 __device__ __forceinline__ void load_classes(const uint8_t *in, bool aligned, const uint8_t *s_cls,
                                              uint64_t base, int valid, uint32_t (&cls)[LEX_IPT / 4],
                                              uint64_t n) {
@@ -285,21 +287,34 @@ __device__ __forceinline__ void load_classes(const uint8_t *in, bool aligned, co
 #undef LEX_ALIGN_CASE
     classes_of_words(wds, s_cls, cls);
   } else if (valid > 0) {
-    // a document's first or last lanes: byte loads at clamped (in-bounds) positions, with no
-    // branch between them, so that they are in flight together (a guarded load per byte waits
-    // for each in turn, and the wave tile's whole count tile, and every count tile that looks
-    // back to it, waits for this lane)
-    const uint64_t last = n - 1;
+    // a document's first or last lanes: three word loads in flight together, as in the interior
+    // (byte loads, or a guarded load per byte, hold up the wave tile's count tile, and with it
+    // every count tile that looks back to it)
+    const uint8_t *q = in + base;
+    const uint32_t off = (uint32_t)((uintptr_t)q & 15);
+    const uint4 *a4 = reinterpret_cast<const uint4 *>(q - off);
+    // index (from a4) of the word holding the document's last byte; q is inside the document
+    const uint32_t lw = (uint32_t)(((uintptr_t)(in + n - 1) >> 4) - ((uintptr_t)q >> 4));
+    const uint4 x0 = a4[0], x1 = a4[min(1u, lw)], x2 = a4[min(2u, lw)];
+    const uint32_t w[12] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w,
+                            x2.x, x2.y, x2.z, x2.w};
+    const uint32_t sh = off & 3;
+    uint32_t wds[8];
+#define LEX_ALIGN_CASE(W0)                                                              \
+  _Pragma("unroll") for (int r = 0; r < 8; ++r) wds[r] =                                \
+      __builtin_amdgcn_alignbyte(w[(W0) + r + 1], w[(W0) + r], sh);
+    switch (off >> 2) {
+      case 0: LEX_ALIGN_CASE(0) break;
+      case 1: LEX_ALIGN_CASE(1) break;
+      case 2: LEX_ALIGN_CASE(2) break;
+      default: LEX_ALIGN_CASE(3) break;
+    }
+#undef LEX_ALIGN_CASE
+    classes_of_words(wds, s_cls, cls);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      uint32_t v = 0;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int j = q * 4 + r;
-        const uint32_t c = s_cls[in[min(base + (uint64_t)j, last)]];
-        v |= (j < valid ? c : 0u) << (8 * r);
-      }
-      cls[q] = v;
+    for (int k = 0; k < 8; ++k) {
+      const int rem = valid - 4 * k;  // bytes of word k inside the document
+      cls[k] &= rem >= 4 ? 0xFFFFFFFFu : (rem <= 0 ? 0u : (1u << (8 * rem)) - 1u);
     }
   } else {
 #pragma unroll
@@ -667,9 +682,13 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   uint32_t labs[LEX_IPT / 4] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t s_end = s_lane;
   uint32_t lab_end = 0;
-  if (valid == LEX_IPT && base + LEX_IPT < n) {
-    // a full lane followed by another byte (every lane but a document's last): no bounds in the
-    // loop. State x class indexes fit 24 bits (at most 256 states and 256 classes).
+  // the lane holding its document's last byte (no byte follows it)
+  const bool doc_last = valid > 0 && base + (uint64_t)valid == n;
+  if (valid > 0) {
+    // every lane walks all its 32 classes, a document's last lane too (class 0 past the end, and
+    // c_next is 0 there), so that no wave runs a second, bounds-checked walk for one lane; the
+    // marks past the document's end are cleared below. State x class indexes fit 24 bits (at most
+    // 256 states and 256 classes).
     uint32_t ent = dfa[__umul24(s_lane, C) + LEX_CLS(0)];
     uint32_t s = ent & 0xFFu;
     uint32_t lab = ent >> 9;
@@ -692,31 +711,26 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
     }
     s_end = s;
     lab_end = lab;
-  } else if (valid > 0) {
-    uint32_t ent = dfa[s_lane * C + LEX_CLS(0)];
-    uint32_t s = ent & 0xFFu;
-    uint32_t lab = ent >> 9;
-    if (((ent >> 8) & 1u) && lab != ign) startm |= 1u;
-#pragma unroll
-    for (int j = 0; j < LEX_IPT; ++j) {
-      if (j < valid) {
-        labs[j >> 2] |= (lab & 0xFFu) << (8 * (j & 3));
-        const bool inner = j + 1 < valid;
-        const bool has_next = inner || (base + j + 1 < n);
-        const uint32_t cn = inner ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
-        const uint32_t ent2 = has_next ? dfa[s * C + cn] : 0u;
-        const bool prod2 = (ent2 >> 8) & 1u;
-        const uint32_t lab2 = ent2 >> 9;
-        if (has_next && prod2 && lab != ign) endm |= 1u << j;
-        if (inner) {
-          if (prod2 && lab2 != ign) startm |= 1u << (j + 1);
-          s = ent2 & 0xFFu;
-          lab = lab2;
-        }
+    if (doc_last) {
+      // tokens start only inside the document, and its last byte ends no token here (the final
+      // token is appended below); its label is the one after byte valid - 1. The state after
+      // that byte is walked again once the count tile's total is out (s_end below).
+      const uint32_t keep = valid == LEX_IPT ? 0xFFFFFFFFu : (1u << valid) - 1u;
+      startm &= keep;
+      endm &= keep >> 1;
+      uint32_t lw;
+      switch ((valid - 1) >> 2) {  // a register array takes constant indexes only
+        case 0: lw = labs[0]; break;
+        case 1: lw = labs[1]; break;
+        case 2: lw = labs[2]; break;
+        case 3: lw = labs[3]; break;
+        case 4: lw = labs[4]; break;
+        case 5: lw = labs[5]; break;
+        case 6: lw = labs[6]; break;
+        default: lw = labs[7]; break;
       }
+      lab_end = (lw >> (8 * ((valid - 1) & 3))) & 0xFFu;
     }
-    s_end = s;
-    lab_end = lab;
   }
   uint32_t lastm = 0;  // CHUNKED: chunk-last positions of this lane
   if (CHUNKED && valid > 0) {
@@ -787,6 +801,15 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
       s_wlast[lane] = xl;
       s_wreset[lane] = (wr || xf) ? 0u : 1u;
     }
+  }
+  // a document's last lane ending inside its 32 bytes: the DFA state after its last byte (the
+  // walk above went on over class 0), walked now that the count tile's total is out
+  if (doc_last && valid < LEX_IPT) {
+    uint32_t s = s_lane;
+#pragma unroll
+    for (int j = 0; j < LEX_IPT - 1; ++j)
+      if (j < valid) s = dfa[s * C + LEX_CLS(j)] & 0xFFu;
+    s_end = s;
   }
   __syncthreads();
 

@@ -139,3 +139,28 @@ def test_dense_tokens_past_the_tile_estimate():
                 assert _tree(got[i]) == _tree(orc.parse(d)), (call, i, len(d))
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("g", ["json", "sexp"])
+def test_document_ends_at_every_lane_offset(ctxs, g):
+    """Prefixes of one document ending at every offset of a lane's 32 bytes, packed back to back
+    (so every document starts at another offset from a 16-byte boundary and ends inside a lane):
+    each document's tokens, verdict and tree must equal the C restatement's."""
+    from alpacc_amd import datagen
+    from oracle.c import coracle
+    b, ctx = ctxs[g]
+    orc = coracle.Oracle(b)
+    base = getattr(datagen, GEN[g])(6000, seed=32).tobytes()
+    lengths = list(range(1, 41)) + list(range(2048 - 33, 2048 + 34)) + [4064 + r for r in range(33)]
+    docs = [base[:n] for n in lengths]
+    print(f"lane-offset ends {g}: {len(docs)} docs", flush=True)
+    lex = ctx.lex_int_batch(docs)
+    par = ctx.parse_int_batch(docs)
+    verdicts = set()
+    for i, d in enumerate(docs):
+        want = orc.lex(d)
+        verdicts.add(want is None)
+        assert _lex_rows(lex[i]) == _lex_rows(want), (g, i, len(d))
+        assert _tree(par[i]) == _tree(orc.parse(d)), (g, i, len(d))
+    if g == "json":  # every prefix of an s-expression lexes
+        assert verdicts == {True, False}, "the cuts should give both verdicts"

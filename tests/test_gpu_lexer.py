@@ -136,9 +136,9 @@ def test_large_document_invariants(ctxs):
 
 @pytest.mark.parametrize("g", ["json", "sexp"])
 def test_unaligned_input(ctxs, g):
-    """Input pointers 1..15 bytes past a 16-byte boundary: lanes inside the document read the
-    aligned 16-byte words around their bytes, lanes at its edges read bytes; tokens and trees must
-    equal the C restatement's."""
+    """Input pointers 1..15 bytes past a 16-byte boundary: every lane reads the aligned 16-byte
+    words around its bytes (at the document's edges clamped to the words holding its first and
+    last byte); tokens must equal the C restatement's."""
     from alpacc_amd import datagen
     from alpacc_amd.runtime import DeviceBuffer
     from oracle.c import coracle
@@ -167,3 +167,46 @@ def test_unaligned_input(ctxs, g):
             buf.free()
             for o in outs:
                 o.free()
+
+
+@pytest.mark.parametrize("g", ["json", "sexp"])
+def test_document_end_at_every_lane_offset(ctxs, g):
+    """The document's last byte at every offset of a lane's 32 bytes (lengths 1-40, around the
+    first wave tile's end and near the second's), the input 0, 5 or 13 bytes past a 16-byte
+    boundary. The last lane walks all its classes like the others, its marks past the end are
+    cleared and the state after its last byte is walked again for the verdict; cuts inside a
+    token change the verdict. Tokens and verdicts must equal the C restatement's."""
+    from alpacc_amd import datagen
+    from alpacc_amd.runtime import DeviceBuffer
+    from oracle.c import coracle
+    bundle, ctx = ctxs[g]
+    orc = coracle.Oracle(bundle)
+    gen = {"json": datagen.json_doc, "sexp": datagen.sexp_doc}[g]
+    base = gen(6000, seed=31)
+    lengths = list(range(1, 41)) + list(range(2048 - 33, 2048 + 34)) + [4064 + r for r in range(33)]
+    assert len(base) >= max(lengths)
+    buf = DeviceBuffer(len(base) + 64)
+    cap = len(base) + 1
+    outs = [DeviceBuffer(4 * cap) for _ in range(3)]
+    verdicts = set()
+    try:
+        for n in lengths:
+            doc = base[:n]
+            want = orc.lex(doc.tobytes())
+            want = None if want is None else [tuple(int(v) for v in t) for t in zip(*want)]
+            verdicts.add(want is None)
+            for off in (0, 5, 13):
+                buf.upload(doc, offset=off)
+                rc, cnt = ctx.lex_device(buf.ptr + off, n, outs[0].ptr, outs[1].ptr, outs[2].ptr,
+                                         cap)
+                got = None
+                if rc == 1:
+                    k, s, e = (o.download(np.uint32, cnt) for o in outs)
+                    got = [tuple(int(v) for v in t) for t in zip(k, s, e)]
+                assert got == want, (g, n, off)
+    finally:
+        buf.free()
+        for o in outs:
+            o.free()
+    if g == "json":  # every prefix of an s-expression lexes
+        assert verdicts == {True, False}, "the cuts should give both verdicts"
