@@ -591,13 +591,27 @@ __device__ void lookback_count(const LexArgs &p, uint32_t tile, int lane, uint32
     const uint64_t zerom = __ballot(flag == 0);
     const uint64_t upto = incm ? (((incm & (~incm + 1)) << 1) - 1) : ~0ull;
     if (zerom & upto) {
-      if (++spins > SPIN_LIMIT || ld_relaxed(p.result + 2) != 0) {
+      // wait on the nearest count tile not yet published, one word per poll, then read the
+      // window again: every waiting count tile reading its whole window (and the shared timeout
+      // word) at each poll piles requests onto the few lines that hold the recent words, and the
+      // loads of the tiles still walking slow down behind them
+      const int zl = __ffsll((unsigned long long)(zerom & upto)) - 1;
+      const uint64_t *zw = p.st_count + (j - zl);
+      bool given_up = false;
+      while ((ld_relaxed64(zw) >> 62) == 0) {
+        ++spins;
+        if (spins > SPIN_LIMIT || ((spins & 63) == 1 && ld_relaxed(p.result + 2) != 0)) {
+          given_up = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      if (given_up) {
         if (lane == 0) st_relaxed(p.result + 2, 1u);
         cnt = 0;
         last = -1;
         return;
       }
-      __builtin_amdgcn_s_sleep(1);
       continue;
     }
     const bool in = (upto >> lane) & 1ull;
