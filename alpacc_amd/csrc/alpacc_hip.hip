@@ -1589,16 +1589,45 @@ __device__ __forceinline__ uint32_t pk_hash_lookup(const PArgs &a, uint64_t di) 
 template <bool BATCH, int KW>
 __device__ __forceinline__ void keys_window(const PArgs &a, uint64_t tile, uint64_t ntiles,
                                             uint32_t W, PosView &pv,
-                                            uint32_t (&sym)[PK_IPT + KW - 1]) {
+                                            uint32_t (&sym)[PK_IPT + KW - 1], bool &raw,
+                                            uint32_t &off) {
   if (tile < ntiles) {
     pv = pos_view<BATCH>(a, tile);
   } else {
     pv = PosView{a.kinds, a.kinds8, 0, 0, 0, 0, false, 0};  // every field set: the view stays in registers
   }
   const int64_t lp0 = pv.p0 + (int64_t)(threadIdx.x % PK_THREADS) * PK_IPT;  // in the document
+  // A tile away from both ends of byte kinds (uniform over its threads): a thread's 7 symbols
+  // are 7 consecutive bytes, read as the three aligned words around them (raw; keys_symbols
+  // takes the bytes out). 16 positions of margin keep the words inside the kinds.
+  raw = KW == 4 && tile < ntiles && pv.kinds8 != nullptr && pv.p0 >= (int64_t)a.q + 4 &&
+        pv.p0 + PK_TILE + 16 <= (int64_t)pv.P;
+  if (raw) {
+    const uintptr_t addr = (uintptr_t)(pv.kinds8 + (lp0 - (int64_t)a.q - 1));
+    off = (uint32_t)(addr & 3);
+    const uint32_t *wp = reinterpret_cast<const uint32_t *>(addr - off);
+    sym[0] = wp[0];
+    sym[1] = wp[1];
+    sym[2] = wp[2];
+#pragma unroll
+    for (int j = 3; j < PK_IPT + KW - 1; ++j) sym[j] = 0u;
+    return;
+  }
+  off = 0;
 #pragma unroll
   for (int j = 0; j < PK_IPT + KW - 1; ++j)
     sym[j] = j < PK_IPT + (int)W - 1 ? pk_symbol(a, pv, lp0 - (int64_t)a.q + j) : 0u;
+}
+
+// The symbols of a window keys_window read raw: bytes off .. off + 6 of its three words.
+template <int KW>
+__device__ __forceinline__ void keys_symbols(const uint32_t (&raw)[PK_IPT + KW - 1], uint32_t off,
+                                             uint32_t (&sym)[PK_IPT + KW - 1]) {
+  const uint32_t lo = __builtin_amdgcn_alignbyte(raw[1], raw[0], off);
+  const uint32_t hi = __builtin_amdgcn_alignbyte(raw[2], raw[1], off);
+#pragma unroll
+  for (int j = 0; j < PK_IPT + KW - 1; ++j)
+    sym[j] = j < 4 ? (lo >> (8 * j)) & 0xFFu : (j < 8 ? (hi >> (8 * (j - 4))) & 0xFFu : 0u);
 }
 
 // KW: the largest key window q + k the instance handles (4 or PK_MAXW); the next iteration's
@@ -1628,18 +1657,24 @@ parse_keys2(PArgs a) {
   bool miss = false;
   PosView npv;
   uint32_t nsym[PK_IPT + KW - 1];
+  bool nraw = false;
+  uint32_t noff = 0;
   if ((uint64_t)blockIdx.x < nsuper)
     keys_window<BATCH, KW>(a, blockIdx.x * (uint64_t)PKK_TILES + threadIdx.x / PK_THREADS, ntiles,
-                           W, npv, nsym);
+                           W, npv, nsym, nraw, noff);
   for (uint64_t sup = blockIdx.x; sup < nsuper; sup += gridDim.x) {
     const uint64_t my_tile = sup * PKK_TILES + threadIdx.x / PK_THREADS;
     const PosView pv = npv;
     uint32_t sym[PK_IPT + KW - 1];
+    if (nraw) {
+      keys_symbols<KW>(nsym, noff, sym);
+    } else {
 #pragma unroll
-    for (int j = 0; j < PK_IPT + KW - 1; ++j) sym[j] = nsym[j];
+      for (int j = 0; j < PK_IPT + KW - 1; ++j) sym[j] = nsym[j];
+    }
     if (sup + gridDim.x < nsuper)
       keys_window<BATCH, KW>(a, (sup + gridDim.x) * PKK_TILES + threadIdx.x / PK_THREADS, ntiles,
-                             W, npv, nsym);
+                             W, npv, nsym, nraw, noff);
     const int64_t lp0 = pv.p0 + (int64_t)(threadIdx.x % PK_THREADS) * PK_IPT;  // in the document
     PTup acc = ptup_id();
     uint32_t idp[PK_IPT / 2] = {0u, 0u};  // entry of every position, two u16 to a word
