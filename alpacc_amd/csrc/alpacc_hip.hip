@@ -135,8 +135,7 @@ struct LexArgs {
   const uint16_t *endo_step;
   const uint16_t *endo_comp;
   const uint8_t *endo_apply;
-  const uint16_t *dfa_step;
-  const uint8_t *state_accept;
+  const uint32_t *dfa_walk;   // [(m+1)*C] lex_emit's walk table (walk_word below)
   uint32_t e, C, m1, init;
   int32_t ignore;
   uint32_t num_tiles;         // wave tiles of LEX_TILE bytes
@@ -639,7 +638,7 @@ __device__ __forceinline__ uint64_t pack_count(uint32_t cnt, int32_t last, uint6
 // DFA_LDS: the DFA table ((m + 1) x C entries) is copied to LDS; else it is read from L2.
 template <bool CHUNKED, bool BATCH, bool DFA_LDS>
 __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
-  __shared__ uint16_t s_dfa[LEX_MAX_DFA];
+  __shared__ uint32_t s_dfa[LEX_MAX_DFA];
   __shared__ uint8_t s_cls[256];
   __shared__ uint32_t s_wcnt[LEX_CT_WAVES];
   __shared__ int32_t s_wlast[LEX_CT_WAVES];
@@ -651,8 +650,11 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   __shared__ uint8_t s_kind[LEX_CT_WAVES][LEX_STAGE];
   const uint32_t C = p.C;
   if (DFA_LDS)
-    for (uint32_t i = threadIdx.x; i < p.m1 * C; i += blockDim.x) s_dfa[i] = p.dfa_step[i];
-  const uint16_t *dfa = DFA_LDS ? s_dfa : p.dfa_step;
+    for (uint32_t i = threadIdx.x; i < p.m1 * C; i += blockDim.x) s_dfa[i] = p.dfa_walk[i];
+  // walk words (the next state's row as a byte offset, kept / boundary / accept bits, label)
+  const uint8_t *dfa = reinterpret_cast<const uint8_t *>(DFA_LDS ? s_dfa : p.dfa_walk);
+#define LEX_WALK(ROW4, CL) (*reinterpret_cast<const uint32_t *>(dfa + (ROW4) + ((CL) << 2)))
+  const uint32_t C4 = C << 2;
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_cls[i] = p.byte_class[i];
   if (threadIdx.x == 0) s_ctile = atomicAdd(p.tile_counter, 1u);
   __syncthreads();
@@ -694,41 +696,44 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   // ---- one walk: token-end mask, token-start mask and the label after every byte
   uint32_t endm = 0, startm = 0;
   uint32_t labs[LEX_IPT / 4] = {0, 0, 0, 0, 0, 0, 0, 0};
-  uint32_t s_end = s_lane;
+  uint32_t acc_end = 0;  // whether the state after the document's last byte accepts
   uint32_t lab_end = 0;
   // the lane holding its document's last byte (no byte follows it)
   const bool doc_last = valid > 0 && base + (uint64_t)valid == n;
   if (valid > 0) {
     // every lane walks all its 32 classes, a document's last lane too (class 0 past the end, and
     // c_next is 0 there), so that no wave runs a second, bounds-checked walk for one lane; the
-    // marks past the document's end are cleared below. State x class indexes fit 24 bits (at most
-    // 256 states and 256 classes).
-    uint32_t ent = dfa[__umul24(s_lane, C) + LEX_CLS(0)];
-    uint32_t s = ent & 0xFFu;
-    uint32_t lab = ent >> 9;
-    bool lab_kept = lab != ign;
-    if (((ent >> 8) & 1u) && lab_kept) startm |= 1u;
+    // marks past the document's end are cleared below. x_j, the word that took byte j, gives the
+    // label of byte j (bits 24-30) and whether it is kept (bit 18); x_{j+1} whether a token
+    // boundary follows byte j (bit 19). A token ends at j where a boundary follows a kept byte and
+    // starts at j + 1 where a boundary precedes a kept byte: two bit masks, ANDed after the walk.
+    uint32_t x = LEX_WALK(__umul24(s_lane, C4), LEX_CLS(0));
+    const uint32_t start0 = (x >> 19) & (x >> 18) & 1u;
+    uint32_t pm = 0, km = 0;
+    uint32_t xs[4];
 #pragma unroll
     for (int j = 0; j < LEX_IPT; ++j) {
-      labs[j >> 2] |= (lab & 0xFFu) << (8 * (j & 3));
-      const uint32_t cn = j + 1 < LEX_IPT ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
-      const uint32_t ent2 = dfa[__umul24(s, C) + cn];
-      const bool prod2 = (ent2 >> 8) & 1u;
-      if (prod2 && lab_kept) endm |= 1u << j;
-      if (j + 1 < LEX_IPT) {
-        const uint32_t lab2 = ent2 >> 9;
-        lab_kept = lab2 != ign;
-        if (prod2 && lab_kept) startm |= 1u << (j + 1);
-        s = ent2 & 0xFFu;
-        lab = lab2;
+      km |= ((x >> 18) & 1u) << j;
+      xs[j & 3] = x;
+      if ((j & 3) == 3) {
+        // the labels of bytes j-3 .. j: byte 3 of their four words
+        const uint32_t t01 = __builtin_amdgcn_perm(xs[1], xs[0], 0x0c0c0703u);
+        const uint32_t t23 = __builtin_amdgcn_perm(xs[3], xs[2], 0x0c0c0703u);
+        labs[j >> 2] = __builtin_amdgcn_perm(t23, t01, 0x05040100u);
       }
+      const uint32_t cn = j + 1 < LEX_IPT ? LEX_CLS((j + 1) & (LEX_IPT - 1)) : c_next;
+      const uint32_t x2 = LEX_WALK(x & 0x3FFFFu, cn);
+      pm |= ((x2 >> 19) & 1u) << j;
+      if (j + 1 < LEX_IPT) x = x2;
     }
-    s_end = s;
-    lab_end = lab;
+    endm = pm & km;
+    startm = start0 | ((pm << 1) & km);
+    acc_end = (x >> 20) & 1u;
+    lab_end = x >> 24;
     if (doc_last) {
       // tokens start only inside the document, and its last byte ends no token here (the final
-      // token is appended below); its label is the one after byte valid - 1. The state after
-      // that byte is walked again once the count tile's total is out (s_end below).
+      // token is appended below); its label is the one after byte valid - 1. Whether the state
+      // after that byte accepts is walked again once the count tile's total is out (below).
       const uint32_t keep = valid == LEX_IPT ? 0xFFFFFFFFu : (1u << valid) - 1u;
       startm &= keep;
       endm &= keep >> 1;
@@ -816,14 +821,14 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
       s_wreset[lane] = (wr || xf) ? 0u : 1u;
     }
   }
-  // a document's last lane ending inside its 32 bytes: the DFA state after its last byte (the
-  // walk above went on over class 0), walked now that the count tile's total is out
+  // a document's last lane ending inside its 32 bytes: whether the state after its last byte
+  // accepts (the walk above went on over class 0), walked now that the count tile's total is out
   if (doc_last && valid < LEX_IPT) {
-    uint32_t s = s_lane;
+    uint32_t x = LEX_WALK(__umul24(s_lane, C4), LEX_CLS(0));
 #pragma unroll
-    for (int j = 0; j < LEX_IPT - 1; ++j)
-      if (j < valid) s = dfa[s * C + LEX_CLS(j)] & 0xFFu;
-    s_end = s;
+    for (int j = 1; j < LEX_IPT - 1; ++j)
+      if (j < valid) x = LEX_WALK(x & 0x3FFFFu, LEX_CLS(j));
+    acc_end = (x >> 20) & 1u;
   }
   __syncthreads();
 
@@ -986,14 +991,15 @@ __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
       total = out + 1;
     }
     if (BATCH) {
-      p.doc_res[2 * tv.doc] = p.state_accept[s_end] ? 1u : 0u;
+      p.doc_res[2 * tv.doc] = acc_end;
       p.doc_res[2 * tv.doc + 1] = total - (uint32_t)tv.tok_base;
     } else {
       p.result[1] = total;
-      p.result[0] = p.state_accept[s_end] ? 1u : 0u;
+      p.result[0] = acc_end;
     }
   }
 }
+#undef LEX_WALK
 
 // Chunked mode: in lex_step every element that ends no token is scattered to prev_size - 1
 // (lexer.fut:108-111, 131-132), the last token of the chunks before; with several such elements
@@ -2519,8 +2525,9 @@ struct alpacc_ctx {
   uint32_t e = 0, C = 0, m1 = 0, init = 0, dead = 0;
   int32_t ignore = -1;
   uint8_t *d_cls = nullptr;
-  uint16_t *d_step = nullptr, *d_comp = nullptr, *d_dfa = nullptr;
-  uint8_t *d_apply = nullptr, *d_accept = nullptr;
+  uint16_t *d_step = nullptr, *d_comp = nullptr;
+  uint32_t *d_walk = nullptr;
+  uint8_t *d_apply = nullptr;
   int lex_tab = 0;       // lex_reduce's table placement (see its TAB)
   bool dfa_lds = true;   // lex_emit's DFA table in LDS
   size_t lex_lds = 0;    // lex_reduce's dynamic LDS
@@ -2916,8 +2923,7 @@ int run_lex(alpacc_ctx *ctx, const uint8_t *d_in, uint64_t n, uint32_t *kinds, u
   a.endo_step = ctx->d_step;
   a.endo_comp = ctx->d_comp;
   a.endo_apply = ctx->d_apply;
-  a.dfa_step = ctx->d_dfa;
-  a.state_accept = ctx->d_accept;
+  a.dfa_walk = ctx->d_walk;
   a.e = ctx->e;
   a.C = ctx->C;
   a.m1 = ctx->m1;
@@ -3068,8 +3074,7 @@ int plan_lex_batch(alpacc_ctx *ctx, const uint8_t *d_in, const uint64_t *offsets
   a.endo_step = ctx->d_step;
   a.endo_comp = ctx->d_comp;
   a.endo_apply = ctx->d_apply;
-  a.dfa_step = ctx->d_dfa;
-  a.state_accept = ctx->d_accept;
+  a.dfa_walk = ctx->d_walk;
   a.e = ctx->e;
   a.C = ctx->C;
   a.m1 = ctx->m1;
@@ -3916,8 +3921,27 @@ alpacc_ctx *alpacc_create(const alpacc_tables *t, int device) {
     ctx->d_step = upload(t->endo_step, (size_t)ctx->e * ctx->C, err);
     ctx->d_comp = upload(t->endo_comp, (size_t)ctx->e * ctx->e, err);
     ctx->d_apply = upload(t->endo_apply, (size_t)ctx->e * ctx->m1, err);
-    ctx->d_dfa = upload(t->dfa_step, (size_t)ctx->m1 * ctx->C, err);
-    ctx->d_accept = upload(t->state_accept, ctx->m1, err);
+    {
+      // lex_emit's walk table, one word per (state, class) of dfa_step: the next state's row
+      // as a byte offset into this table (bits 0-17: at most 255 x 256 x 4), whether the label of
+      // the byte is kept (not the ignored terminal; bit 18), whether a token boundary falls before
+      // the byte (the produce bit; bit 19), whether the next state accepts (bit 20) and the label
+      // (bits 24-30).
+      const size_t nw = (size_t)ctx->m1 * ctx->C;
+      std::vector<uint32_t> walk(nw);
+      for (size_t i = 0; i < nw && err.empty(); ++i) {
+        const uint32_t e = t->dfa_step[i];
+        const uint32_t ns = e & 0xFFu, prod = (e >> 8) & 1u, lab = e >> 9;
+        if (ns >= ctx->m1) {
+          err = "lexer tables out of range";
+          break;
+        }
+        const bool kept = ctx->ignore < 0 || lab != (uint32_t)ctx->ignore;
+        walk[i] = ns * ctx->C * 4u | (kept ? 1u << 18 : 0u) | prod << 19 |
+                  (t->state_accept[ns] ? 1u << 20 : 0u) | lab << 24;
+      }
+      if (err.empty()) ctx->d_walk = upload(walk.data(), nw, err);
+    }
     if (!err.empty()) {
       alpacc_destroy(ctx);
       return nullptr;
@@ -3929,7 +3953,7 @@ alpacc_ctx *alpacc_create(const alpacc_tables *t, int device) {
   }
   if (ctx->has_lexer)
     ctx->table_bytes += 256 + (uint64_t)ctx->e * ctx->C * 2 + (uint64_t)ctx->e * ctx->e * 2 +
-                        (uint64_t)ctx->e * ctx->m1 + (uint64_t)ctx->m1 * ctx->C * 2 + ctx->m1;
+                        (uint64_t)ctx->e * ctx->m1 + (uint64_t)ctx->m1 * ctx->C * 4;
   if (t->num_productions > 0 && (t->pair_index || t->pair_hash_len)) {
     ctx->has_parser = true;
     ctx->q = t->q;
@@ -3953,7 +3977,7 @@ void alpacc_destroy(alpacc_ctx *ctx) {
   for (auto &s : ctx->scratch)
     if (s.first) (void)hipFree(s.first);
   void *tabs[] = {ctx->d_cls,    ctx->d_step,     ctx->d_comp,   ctx->d_apply,
-                  ctx->d_dfa,    ctx->d_accept,   ctx->d_pair_index, ctx->d_index8,
+                  ctx->d_walk,   ctx->d_pair_index, ctx->d_index8,
                   ctx->d_hkeys,  ctx->d_hvals,
                   ctx->d_einfo,  ctx->d_stackw,   ctx->d_prodw,  ctx->d_ecan};
   for (void *p : tabs)
