@@ -20,7 +20,8 @@ The closure here composes every pair of reachable endomorphisms (not only the pa
 "connected" worklist visits, ParallelLexing.hs:222-264), and every composition that maps all states
 to the dead state is identified with the dead endomorphism. Both differences only change internal
 endomorphism numbering: the value a scan produces at every position (token, accept, produce of the
-state reached from the initial state) is the same.
+state reached from the initial state) is the same. The worklist, run on the same DFA, reaches the
+same non-dead endomorphisms (tests/test_endo_closure.py), so the full closure adds none.
 """
 from __future__ import annotations
 
@@ -59,6 +60,7 @@ class EndoTables:
     # raw functions (rows of dead + DFA states), for validation
     functions: np.ndarray          # int32 [e, m+1]; rows 0/1 are identity/dead
     byte_endo: np.ndarray          # int32 [256] endomorphism index of each byte
+    produces: np.ndarray = None    # bool [e, m+1]: whether the move from each state produces
 
     def encode(self, idx: int) -> int:
         return (int(idx)
@@ -205,7 +207,7 @@ def build_endo_tables(ld: LexerDFA, terminal_id: dict, num_terminals: int, dead_
         transitions_to_states=np.zeros(256, dtype=np.uint32),
         accept=endo_accept, identity=0, dead_terminal=dead_terminal,
         ignore_terminal=ignore_terminal, endo_terminal=endo_terminal, endo_produce=endo_produce,
-        functions=Fa, byte_endo=byte_endo)
+        functions=Fa, byte_endo=byte_endo, produces=Pa)
     enc = np.array([tables.encode(i) for i in range(e)], dtype=np.uint32)
     tables.compositions = enc[comp.reshape(-1)].astype(np.uint32)
     tables.transitions_to_states = enc[byte_endo].astype(np.uint32)
