@@ -135,7 +135,8 @@ struct LexArgs {
   const uint16_t *endo_step;
   const uint16_t *endo_comp;
   const uint8_t *endo_apply;
-  const uint32_t *dfa_walk;   // [(m+1)*C] lex_emit's walk table (walk_word below)
+  const uint32_t *dfa_walk;   // [(m+1)*C] lex_emit's walk words (laid out where alpacc_create
+                              // builds them from dfa_step)
   uint32_t e, C, m1, init;
   int32_t ignore;
   uint32_t num_tiles;         // wave tiles of LEX_TILE bytes
@@ -635,7 +636,7 @@ __device__ __forceinline__ uint64_t pack_count(uint32_t cnt, int32_t last, uint6
 // chunked mode; the last element of every chunk but the last ends no token (i != n - 1 in
 // lex_step, lexer.fut:104-107), spans are shifted as lex_step does, and every chunk's last
 // element is recorded in chunk_info for lex_chunk_fixup.
-// DFA_LDS: the DFA table ((m + 1) x C entries) is copied to LDS; else it is read from L2.
+// DFA_LDS: the walk words (dfa_walk, (m + 1) x C) are copied to LDS; else they are read from L2.
 template <bool CHUNKED, bool BATCH, bool DFA_LDS>
 __global__ void __launch_bounds__(LEX_BLOCK) lex_emit(LexArgs p) {
   __shared__ uint32_t s_dfa[LEX_MAX_DFA];
