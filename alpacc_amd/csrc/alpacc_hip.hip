@@ -1691,7 +1691,10 @@ parse_keys2(PArgs a) {
     for (int i = 0; i < PK_IPT; ++i) {
       const int64_t p = lp0 + i;
       tlr[i] = 0;
-      if (p >= (int64_t)pv.P) continue;
+      if (p >= (int64_t)pv.P) {  // past the document: expands to nothing (parse_expand2)
+        idp[i >> 1] |= (uint32_t)PK_NO_ENTRY << (16 * (i & 1));
+        continue;
+      }
       uint64_t di = 0;
       bool unknown = false;
 #pragma unroll
@@ -1757,18 +1760,11 @@ parse_keys2(PArgs a) {
     } else {
       miss |= wmiss;
     }
-    // slots are tile-major: tile * PK_TILE + position in the tile
+    // slots are tile-major: tile * PK_TILE + position in the tile; every slot of a tile is written,
+    // those past the document's positions with PK_NO_ENTRY, so that parse_expand2 tells them
+    // without the document's bounds
     const uint64_t pbase = sup * PKK_TILES * PK_TILE + (uint64_t)threadIdx.x * PK_IPT;
-    const uint64_t pend = my_tile * PK_TILE + (pv.P > (uint64_t)pv.p0 ? pv.P - (uint64_t)pv.p0 : 0);
-    if (pbase + PK_IPT <= pend) {
-      uint2 packed;
-      packed.x = idp[0];
-      packed.y = idp[1];
-      *reinterpret_cast<uint2 *>(a.ent + pbase) = packed;
-    } else {
-      for (int i = 0; i < PK_IPT; ++i)
-        if (pbase + i < pend) a.ent[pbase + i] = (uint16_t)(idp[i >> 1] >> (16 * (i & 1)));
-    }
+    if (my_tile < ntiles) *reinterpret_cast<uint2 *>(a.ent + pbase) = make_uint2(idp[0], idp[1]);
     const PTup incl = ptup_wave_incl(acc, lane);
     if (lane == WAVE - 1) s_w[w] = incl;
     __syncthreads();
@@ -2060,13 +2056,12 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
     if (tile >= *a.ntiles_dev || !batch_elems_fit(a)) return;
   }
   const uint64_t pbase = tile * PK_TILE + (uint64_t)threadIdx.x * PK_IPT;  // slot
-  // the lane's four entries, loaded before anything else the tile's bounds depend on (the
-  // document descriptor, the device token count): the entry array holds every launched tile
-  // whole, and entries past the document's end are not used (pend below)
+  // the lane's four entries: the entry array holds every tile whole, with PK_NO_ENTRY past the
+  // document's positions (parse_keys2), so the entries alone tell which positions are there, and
+  // the document's descriptor (and the device token count) are needed only after the block scan
   const uint2 packed = *reinterpret_cast<const uint2 *>(a.ent + pbase);
   const PosView pv = pos_view<BATCH>(a, tile);
   const int subst = pv.subst;
-  const uint64_t pend = tile * PK_TILE + (pv.P > (uint64_t)pv.p0 ? pv.P - (uint64_t)pv.p0 : 0);
   // batched: the first node of the tile's document and of the next one, for the chunk words
   uint32_t doc_node0 = 0, next_node0 = 0xFFFFFFFFu;
   if (BATCH) {
@@ -2091,7 +2086,7 @@ parse_expand2(PArgs a, const S *starts, const S *ends,
   PTup acc = ptup_id();
 #pragma unroll
   for (int i = 0; i < PK_IPT; ++i) {
-    const bool has = pbase + i < pend && ids[i] != PK_NO_ENTRY;
+    const bool has = ids[i] != PK_NO_ENTRY;
     inf[i] = has ? s_einfo[ids[i]] : EntInfo{0, 0, 0, INT16_MAX, 0, 0, 0, 0};
     tlr[i] = has ? s_can[ids[i]] : 0u;
     acc = ptup_op(acc, ptup_of(inf[i]));
