@@ -4,7 +4,8 @@ The reference's Futhark artifact exposes (src/Alpacc/Generator/Futhark/Generator
   * ``lex_int``   : bytes -> (terminal ids, starts, ends), empty arrays when rejected;
   * ``lex``       : the same with terminal names (lexer.fut:180-191);
   * ``parse_int`` : bytes -> opt [(parent, #terminal t (i, j) | #production p)];
-  * ``parse``     : the same with names (parser.fut:338-350).
+  * ``parse``     : the same with names (parser.fut:338-350);
+  * ``format_lex`` / ``format_parse`` print those results in the README's form (README.md:84-106).
 Here ``lex_int``/``parse_int`` return ``None`` for the reference's ``#none`` (rejection) instead of
 empty arrays, because "accepted with zero tokens" and "rejected" must stay distinguishable.
 
@@ -832,3 +833,27 @@ def tree_tuples(r):
         else:
             out.append((p, False, nd, 0, 0))
     return out
+
+
+def format_parse(tree) -> str:
+    """The printed form of a ``parse`` result as README.md:84-106 shows it: ``#some [(parent,
+    #production (#name)), (parent, #terminal (#name) (start, end)), ...]``, one node a line, or
+    ``#none`` for a rejected input (``tree`` is None)."""
+    if tree is None:
+        return "#none"
+    items = []
+    for parent, node in tree:
+        if node[0] == "terminal":
+            _, name, (s, e) = node
+            items.append(f"({parent}, #terminal (#{name}) ({s}, {e}))")
+        else:
+            items.append(f"({parent}, #production (#{node[1]}))")
+    return "#some [" + ",\n       ".join(items) + "]"
+
+
+def format_lex(tokens) -> str:
+    """The same form for a ``lex`` result (lexer.fut:180-191: opt [(terminal, (start, end))]):
+    ``#some [(#name, (start, end)), ...]`` or ``#none``."""
+    if tokens is None:
+        return "#none"
+    return "#some [" + ",\n       ".join(f"(#{t}, ({s}, {e}))" for t, (s, e) in tokens) + "]"

@@ -128,7 +128,9 @@ def readme_tree() -> dict:
     for m in re.finditer(r'\((\d+), #(production|terminal) \(#(\w+)\)(?: \((\d+), (\d+)\))?\)', blk):
         nodes.append({"parent": int(m.group(1)), "kind": m.group(2), "name": m.group(3),
                       "span": [int(m.group(4)), int(m.group(5))] if m.group(4) else None})
-    return {"input": "12+(3*4)", "grammar": "arithmetic", "nodes": nodes}
+    # the printed block itself, whitespace collapsed: the README's form of a parse result
+    text = " ".join(blk.split())
+    return {"input": "12+(3*4)", "grammar": "arithmetic", "nodes": nodes, "text": text}
 
 
 def main():

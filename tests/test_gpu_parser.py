@@ -40,6 +40,13 @@ def test_readme_tree(ctxs):
     assert names[6] == (5, ("terminal", "literal_5", (2, 3)))
     assert names[15] == (14, ("terminal", "literal_4", (5, 6)))
     assert names[-1] == (5, ("production", "Empty"))
+    # printed as the README prints it (README.md:84-106, whitespace collapsed)
+    import json
+    from alpacc_amd.runtime import format_parse
+    kat = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "haskell_vectors.json")))["readme_tree"]
+    assert " ".join(format_parse(names).split()) == kat["text"]
+    assert format_parse(ctx.parse(b"12+(3*")) == "#none"
 
 
 @pytest.mark.parametrize("g", ["json", "arithmetic", "sexp"])

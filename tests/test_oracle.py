@@ -56,6 +56,31 @@ def test_readme_tree_all_oracles(bundles):
     assert _tree_tuples(*coracle.Oracle(b).parse(data)) == seq
 
 
+def test_readme_printed_form(bundles):
+    """format_parse prints the README's block (README.md:84-106, whitespace collapsed) for the
+    sequential parser's tree of 12+(3*4); format_lex prints the tokens the same way."""
+    import json
+    import re
+    from alpacc_amd.runtime import format_lex, format_parse
+    from oracle import sequential
+    from oracle.c import coracle
+    kat = json.load(open(os.path.join(ROOT, "tests", "golden", "haskell_vectors.json")))["readme_tree"]
+    b = bundles["arithmetic"]
+    data = kat["input"].encode()
+    tnames, pnames = b.terminal_names, b.parser.production_names
+    tree = [(p, ("terminal", tnames[i], (s, e)) if t else ("production", pnames[i]))
+            for p, t, i, s, e in sequential.parse(b, data)]
+    assert " ".join(format_parse(tree).split()) == kat["text"]
+    assert format_parse(None) == "#none" and format_lex(None) == "#none"
+    k, s, e = coracle.Oracle(b).lex(data)
+    toks = [(tnames[int(x)], (int(y), int(z))) for x, y, z in zip(k, s, e)]
+    text = format_lex(toks)
+    assert text.startswith("#some [(#num, (0, 2)),\n       (#literal_5, (2, 3)),")
+    assert re.findall(r"\(#(\w+), \((\d+), (\d+)\)\)", text) == \
+        [(t, str(x), str(y)) for t, (x, y) in toks]
+    assert [nd["span"] for nd in kat["nodes"] if nd["span"]] == [[x, y] for _, (x, y) in toks]
+
+
 @pytest.mark.parametrize("g,maxlen", [("sexp", 5), ("arithmetic", 4), ("json", 2)])
 def test_lexers_agree_exhaustive(bundles, g, maxlen):
     """tokenize (DFA.hs) == literal lexer.fut restatement == C oracle on every string up to
