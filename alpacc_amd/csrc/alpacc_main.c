@@ -7,12 +7,15 @@
  * header written by alpacc_amd.gen.emit_c; compile with
  *   -DALPACC_TABLES_HEADER='"json_tables.h"' -DALPACC_PREFIX=json
  *
- * usage: alpacc_<grammar> [lex|parse|test] [--names] < input
+ * usage: alpacc_<grammar> [lex|parse|test] [--names|--readme] < input
  *   lex   : "start end terminal" per token (the reference's WriteAscii format)
  *   parse : "parent T terminal start end" or "parent P production" per node, in preorder
  *   test  : the artifact's `test` entry (futhark/test.fut): wire-format inputs on stdin,
  *           wire-format outputs on stdout
  * --names prints terminal/production names instead of ids.
+ * --readme prints the result as README.md:84-106 shows a parse: "#some [(0, #production (#E0_0)),
+ *   (2, #terminal (#num) (0, 2)), ...]" one item a line (for lex "(#num, (0, 2))" items), or
+ *   "#none" for a rejected input.
  */
 #include <stdint.h>
 #include <stdio.h>
@@ -64,14 +67,15 @@ static int fail(const char *what, alpacc_ctx *ctx) {
 }
 
 int main(int argc, char **argv) {
-  int parse = 0, names = 0, test = 0;
+  int parse = 0, names = 0, test = 0, readme = 0;
   for (int i = 1; i < argc; ++i) {
     if (!strcmp(argv[i], "parse")) parse = 1;
     else if (!strcmp(argv[i], "lex")) parse = 0;
     else if (!strcmp(argv[i], "test")) test = 1;
     else if (!strcmp(argv[i], "--names")) names = 1;
+    else if (!strcmp(argv[i], "--readme")) readme = 1;
     else {
-      fprintf(stderr, "usage: %s [lex|parse|test] [--names] < input\n", argv[0]);
+      fprintf(stderr, "usage: %s [lex|parse|test] [--names|--readme] < input\n", argv[0]);
       return 2;
     }
   }
@@ -110,11 +114,15 @@ int main(int argc, char **argv) {
       if (!h || alpacc_copy_to_host(h, d, 3 * cap * sizeof(uint32_t), NULL) != 0)
         return fail("download failed", ctx);
       for (uint64_t i = 0; i < T; ++i) {
-        if (names)
+        if (readme)
+          printf("%s(#%s, (%u, %u))", i ? ",\n       " : "#some [", TERMINAL_NAMES[h[i]], h[cap + i],
+                 h[2 * cap + i]);
+        else if (names)
           printf("%u %u %s\n", h[cap + i], h[2 * cap + i], TERMINAL_NAMES[h[i]]);
         else
           printf("%u %u %u\n", h[cap + i], h[2 * cap + i], h[i]);
       }
+      if (readme) printf(T ? "]\n" : "#some []\n");
       free(h);
     }
     alpacc_device_free(d);
@@ -137,7 +145,13 @@ int main(int argc, char **argv) {
         const uint32_t *par = h, *node = h + cap, *span = h + 2 * cap;
         for (uint64_t i = 0; i < N; ++i) {
           const uint32_t id = node[i] & ~ALPACC_NODE_TERMINAL;
-          if (node[i] & ALPACC_NODE_TERMINAL) {
+          if (readme) {
+            printf("%s(%u, ", i ? ",\n       " : "#some [", par[i]);
+            if (node[i] & ALPACC_NODE_TERMINAL)
+              printf("#terminal (#%s) (%u, %u))", TERMINAL_NAMES[id], span[2 * i], span[2 * i + 1]);
+            else
+              printf("#production (#%s))", PRODUCTION_NAMES[id]);
+          } else if (node[i] & ALPACC_NODE_TERMINAL) {
             if (names)
               printf("%u T %s %u %u\n", par[i], TERMINAL_NAMES[id], span[2 * i], span[2 * i + 1]);
             else
@@ -148,12 +162,14 @@ int main(int argc, char **argv) {
             printf("%u P %u\n", par[i], id);
           }
         }
+        if (readme) printf(N ? "]\n" : "#some []\n");
         free(h);
       }
       alpacc_device_free(d);
       break;
     }
   }
+  if (readme && rc != ALPACC_ACCEPT) printf("#none\n");
   fflush(stdout);
   alpacc_device_free(d_in);
   alpacc_destroy(ctx);

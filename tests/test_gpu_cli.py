@@ -59,6 +59,37 @@ def test_readme_example(bundles):
 
 
 @pytest.mark.parametrize("g", ["json", "arithmetic", "sexp"])
+def test_readme_form(bundles, g):
+    """--readme prints what format_lex / format_parse print for the oracle's result (README.md:84-106
+    for the arithmetic example, whose block is a fixture), "#none" when rejected."""
+    import json
+    from alpacc_amd import datagen
+    from alpacc_amd.runtime import format_lex, format_parse
+    from oracle import sequential
+    b = bundles[g]
+    tn, pn = b.terminal_names, b.parser.production_names
+    gen = {"json": datagen.json_doc, "arithmetic": datagen.arith_doc, "sexp": datagen.sexp_doc}[g]
+    docs = [gen(3000, seed=11).tobytes(), b"((" if g != "arithmetic" else b"1+", b""]
+    if g == "arithmetic":
+        kat = json.load(open(os.path.join(ROOT, "tests", "golden",
+                                          "haskell_vectors.json")))["readme_tree"]
+        rc, out = _run(g, ["parse", "--readme"], kat["input"].encode())
+        assert rc == 0 and " ".join(out.split()) == kat["text"]
+        docs.append(kat["input"].encode())
+    for d in docs:
+        toks = sequential.lex(b, d)
+        want = format_lex(None if toks is None else [(tn[k], (s, e)) for k, s, e in toks])
+        rc, out = _run(g, ["lex", "--readme"], d)
+        assert (rc == 0) == (toks is not None) and out == want + "\n", (g, len(d))
+        tree = sequential.parse(b, d)
+        want = format_parse(None if tree is None else [
+            (p, ("terminal", tn[i], (s, e)) if t else ("production", pn[i]))
+            for p, t, i, s, e in tree])
+        rc, out = _run(g, ["parse", "--readme"], d)
+        assert (rc == 0) == (tree is not None) and out == want + "\n", (g, len(d))
+
+
+@pytest.mark.parametrize("g", ["json", "arithmetic", "sexp"])
 def test_documents_and_rejections(bundles, g):
     from alpacc_amd import datagen
     b = bundles[g]
