@@ -30,7 +30,7 @@ import time
 import numpy as np
 
 DEFAULTS = {"graph_mode": 1, "lex_blocks_per_cu": 2, "keys_blocks_per_cu": 2,
-            "search_max_blocks": 65536, "search_block_threads": 256, "brackets_per_kib": 256,
+            "search_max_blocks": 16384, "search_block_threads": 256, "brackets_per_kib": 256,
             "lexer_tables_in_lds": 1}
 
 SPACE = (
@@ -38,7 +38,7 @@ SPACE = (
     ("lex_blocks_per_cu", (1, 2, 3, 4)),
     ("keys_blocks_per_cu", (1, 2, 4)),
     ("search_block_threads", (256, 512, 1024)),
-    ("search_max_blocks", (8192, 32768, 65536, 262144)),
+    ("search_max_blocks", (4096, 8192, 16384, 65536)),
 )
 
 

@@ -2562,8 +2562,9 @@ struct alpacc_ctx {
   // key positions per input byte the batched expansion's grid is sized for (grows likewise)
   double p_per_byte = 0.25;
   // launch shapes (alpacc_set_tuning): persistent lex_reduce / parse_keys2 blocks per CU, and the
-  // largest grid of the search kernels
-  uint32_t lex_bpc = 2, keys_bpc = 2, search_cap = 1u << 16, search_bs = 256;
+  // largest grid of the search kernels (16384 workgroups of 256 threads: 8 per SIMD's worth of
+  // waves several times over; larger grids measured slower on a 256 MiB JSON document)
+  uint32_t lex_bpc = 2, keys_bpc = 2, search_cap = 1u << 14, search_bs = 256;
   uint32_t *h_small = nullptr;  // pinned host words for results
   // profiling: HIP events on the call's stream around each stage
   bool prof = false;

@@ -206,7 +206,7 @@ int alpacc_test_force_timeout(alpacc_ctx *ctx, int on);
  *   lex_blocks_per_cu : persistent lex_reduce workgroups per CU (1..8, default 2)
  *   keys_blocks_per_cu: persistent parse_keys2 workgroups per CU (1..8, default 2)
  *   search_max_blocks : largest grid of bracket_match / parse_parents, in workgroups of 256
- *                       threads (64..2^20, default 65536)
+ *                       threads (64..2^20, default 16384)
  *   search_block_threads: workgroup size of bracket_match / parse_parents (256, 512 or 1024;
  *                       default 256)
  *   brackets_per_kib  : brackets per KiB of input the batched bracket scratch is first sized for
