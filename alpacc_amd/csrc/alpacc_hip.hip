@@ -8,7 +8,8 @@
 //             Blelloch scan composes the 64 lane endomorphisms with the (endo x endo) table.
 //           lex_scan_* / lex_tile_states - exclusive composition of the tile endomorphisms and
 //             the DFA state at every tile start (n / 2048 elements).
-//           lex_emit - lanes walk the plain lexer DFA from their known start states once,
+//           lex_emit - lanes walk the lexer DFA from their known start states once (one walk
+//             word per (state, class): next row, kept / boundary / accept bits, label),
 //             scan (token count, last token start) across the block and, by a decoupled
 //             look-back, across blocks, and write (kind, start, end) of every non-ignored token
 //             through LDS staging, coalesced.
